@@ -1,0 +1,1671 @@
+// bxg_api.hip -- host runtime of the MI355X sparse voxel map behind the C ABI
+// declared in include/bonxai_gpu.h.
+//
+// A context owns one HIP stream (default priority), the scratch of the scan
+// and bulk-op pipelines, pinned staging, events and the memory budget of its
+// maps.  A map owns its device node pools (root directory + leaf pool) and a
+// device header (MapDev) that every kernel reads, so pools can grow between
+// calls without re-capturing graphs.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "bxg_kernels.hip"
+#include "../../include/bonxai_gpu.h"
+
+using namespace bxg;
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+constexpr int kStageSlots = 8;
+
+struct PinnedSlot {
+  void* host = nullptr;
+  uint64_t cap = 0;
+  hipEvent_t done = nullptr;
+};
+
+struct Prof {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct bxg_map;
+
+struct bxg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;
+  uint64_t used = 0;
+  std::string err;
+  std::recursive_mutex mu;
+  int cu_count = 0;
+  uint64_t hbm_bytes = 0;
+  std::string name;
+
+  // scan pipeline scratch
+  uint64_t scan_cap = 0;
+  ScanScratch sc{};
+  uint32_t* d_counters = nullptr;
+  ScanDesc* d_desc = nullptr;
+  BatchHdr* d_hdr = nullptr;
+  uint32_t desc_cap = 0;
+  PinnedSlot stage[kStageSlots];
+  int stage_next = 0;
+
+  // bulk-op scratch
+  uint64_t op_cap = 0;
+  uint64_t* op_keys = nullptr;
+  uint32_t* op_first = nullptr;
+  uint32_t* op_last = nullptr;
+  uint32_t op_mask = 0;
+  uint32_t* op_tpos = nullptr;
+  uint8_t* op_prebit = nullptr;
+
+  // device staging of host-memory calls
+  void* dstage[4] = {nullptr, nullptr, nullptr, nullptr};
+  uint64_t dstage_cap[4] = {0, 0, 0, 0};
+
+  uint64_t* d_small = nullptr;  // 8 words of results
+  uint64_t* h_small = nullptr;  // pinned mirror
+
+  hipEvent_t ev[64] = {};
+  bool profiling = false;
+  bool counting = false;
+  std::vector<Prof> prof_pending;
+  std::vector<hipEvent_t> ev_free;
+  std::map<std::string, std::pair<uint64_t, double>> prof_totals;
+
+  std::vector<bxg_map*> maps;
+};
+
+struct bxg_map {
+  bxg_ctx* ctx = nullptr;
+  MapDev h{};
+  MapDev* d = nullptr;
+  MapCounters* ctr = nullptr;
+  MapCounters* h_ctr = nullptr;  // pinned mirror of the counters
+  hipEvent_t ev_ctr = nullptr;   // completion of the last counter copy
+  bool ctr_pending = false;
+  uint32_t root_cap = 0;
+  uint32_t leaf_cap = 0;
+  uint64_t bytes = 0;
+  uint64_t known_leaves = 0;
+  uint64_t known_roots = 0;
+  int4* d_extra = nullptr;  // rays due from addHit/MissPoint
+  uint32_t extra_cap = 0;
+  std::vector<int4> extra;
+};
+
+struct bxg_graph {
+  bxg_ctx* ctx = nullptr;
+  uint32_t max_scans = 0;
+  uint64_t max_points = 0;
+  ScanScratch sc{};
+  uint32_t* d_counters = nullptr;
+  ScanDesc* d_desc = nullptr;
+  BatchHdr* d_hdr = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<bxg_map*> maps;  // maps of the loaded frame
+  bool loaded = false;
+};
+
+namespace {
+
+int fail(bxg_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(bxg_ctx* c, hipError_t e, const char* what) {
+  return fail(c, BXG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define BXG_HIP(c, expr)                                 \
+  do {                                                   \
+    hipError_t e_ = (expr);                              \
+    if (e_ != hipSuccess) return hip_fail(c, e_, #expr); \
+  } while (0)
+
+#define BXG_TRY(expr)         \
+  do {                        \
+    int r_ = (expr);          \
+    if (r_ != BXG_OK) return r_; \
+  } while (0)
+
+uint64_t pow2_at_least(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+uint32_t nblocks(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+hipEvent_t ev_take(bxg_ctx* c) {
+  if (!c->ev_free.empty()) {
+    hipEvent_t e = c->ev_free.back();
+    c->ev_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+// launch on the context stream, bracketed by events when profiling
+template <typename... KArgs, typename... Args>
+int launch(bxg_ctx* c, const char* name, void (*k)(KArgs...), uint32_t grid, uint32_t block,
+           Args... args) {
+  if (grid == 0) return BXG_OK;
+  hipEvent_t a = nullptr;
+  if (c->profiling) {
+    a = ev_take(c);
+    hipEventRecord(a, c->stream);
+  }
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, c->stream, args...);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(c, e, name);
+  if (c->profiling) {
+    hipEvent_t b = ev_take(c);
+    hipEventRecord(b, c->stream);
+    c->prof_pending.push_back({name, a, b});
+  }
+  return BXG_OK;
+}
+
+int dmalloc(bxg_ctx* c, void** p, uint64_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 64;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return hip_fail(c, e, "hipMalloc");
+  }
+  return BXG_OK;
+}
+
+void dfree(void* p) {
+  if (p) hipFree(p);
+}
+
+int fill32(bxg_ctx* c, uint32_t* p, uint64_t n, uint32_t v) {
+  if (!n) return BXG_OK;
+  const uint32_t g = (uint32_t)std::min<uint64_t>(nblocks(n), 4096);
+  return launch(c, "k_fill_u32", k_fill_u32, g, kBlock, p, n, v);
+}
+int fill64(bxg_ctx* c, uint64_t* p, uint64_t n, uint64_t v) {
+  if (!n) return BXG_OK;
+  const uint32_t g = (uint32_t)std::min<uint64_t>(nblocks(n), 4096);
+  return launch(c, "k_fill_u64", k_fill_u64, g, kBlock, p, n, v);
+}
+
+int ctx_sync(bxg_ctx* c) {
+  BXG_HIP(c, hipStreamSynchronize(c->stream));
+  return BXG_OK;
+}
+
+// device staging buffer `slot` of at least `bytes`
+int dstage(bxg_ctx* c, int slot, uint64_t bytes, void** out) {
+  if (c->dstage_cap[slot] < bytes) {
+    BXG_TRY(ctx_sync(c));
+    dfree(c->dstage[slot]);
+    c->dstage[slot] = nullptr;
+    c->dstage_cap[slot] = 0;
+    const uint64_t cap = std::max<uint64_t>(pow2_at_least(bytes), 1 << 16);
+    BXG_TRY(dmalloc(c, &c->dstage[slot], cap));
+    c->dstage_cap[slot] = cap;
+  }
+  *out = c->dstage[slot];
+  return BXG_OK;
+}
+
+// pinned host staging for stream-ordered uploads: the slot is reused only
+// once its previous copy has completed
+int pinned_slot(bxg_ctx* c, uint64_t bytes, PinnedSlot** out) {
+  PinnedSlot& s = c->stage[c->stage_next];
+  c->stage_next = (c->stage_next + 1) % kStageSlots;
+  if (s.done) BXG_HIP(c, hipEventSynchronize(s.done));
+  if (s.cap < bytes) {
+    if (s.host) hipHostFree(s.host);
+    s.host = nullptr;
+    const uint64_t cap = std::max<uint64_t>(pow2_at_least(bytes), 4096);
+    BXG_HIP(c, hipHostMalloc(&s.host, cap, hipHostMallocDefault));
+    s.cap = cap;
+  }
+  if (!s.done) BXG_HIP(c, hipEventCreateWithFlags(&s.done, hipEventDisableTiming));
+  *out = &s;
+  return BXG_OK;
+}
+
+// stream-ordered upload of a small host structure through pinned staging
+int upload(bxg_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!bytes) return BXG_OK;
+  PinnedSlot* s = nullptr;
+  BXG_TRY(pinned_slot(c, bytes, &s));
+  std::memcpy(s->host, src, bytes);
+  BXG_HIP(c, hipMemcpyAsync(dst, s->host, bytes, hipMemcpyHostToDevice, c->stream));
+  BXG_HIP(c, hipEventRecord(s->done, c->stream));
+  return BXG_OK;
+}
+
+int h2d_sync(bxg_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!bytes) return BXG_OK;
+  BXG_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return ctx_sync(c);
+}
+int d2h_sync(bxg_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!bytes) return BXG_OK;
+  BXG_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  return ctx_sync(c);
+}
+
+// ------------------------------------------------------------ map pools
+
+uint64_t leaf_bytes(const bxg_map* m) {
+  return (uint64_t)m->h.leaf_words * 8 + (uint64_t)m->h.leaf_size * m->h.value_bytes + sizeof(int4);
+}
+uint64_t root_bytes(const bxg_map* m) { return 8 + (uint64_t)m->h.inner_size * 4; }
+
+int push_header(bxg_map* m) { return upload(m->ctx, m->d, &m->h, sizeof(MapDev)); }
+
+int budget_check(bxg_ctx* c, uint64_t more) {
+  if (c->budget && c->used + more > c->budget)
+    return fail(c, BXG_ERR_BUDGET,
+                "memory budget exceeded: " + std::to_string(c->used + more) + " > " +
+                    std::to_string(c->budget));
+  return BXG_OK;
+}
+
+// read the device counters of a map (blocking)
+int read_counters(bxg_map* m) {
+  bxg_ctx* c = m->ctx;
+  BXG_HIP(c, hipMemcpyAsync(m->h_ctr, m->ctr, sizeof(MapCounters), hipMemcpyDeviceToHost, c->stream));
+  BXG_TRY(ctx_sync(c));
+  m->ctr_pending = false;
+  m->known_leaves = std::min<uint64_t>(m->h_ctr->leaf_count, m->leaf_cap);
+  m->known_roots = m->h_ctr->root_count;
+  return BXG_OK;
+}
+
+// enqueue a counter snapshot after a mutating call (polled lazily)
+int snapshot_counters(bxg_map* m) {
+  bxg_ctx* c = m->ctx;
+  BXG_HIP(c, hipMemcpyAsync(m->h_ctr, m->ctr, sizeof(MapCounters), hipMemcpyDeviceToHost, c->stream));
+  BXG_HIP(c, hipEventRecord(m->ev_ctr, c->stream));
+  m->ctr_pending = true;
+  return BXG_OK;
+}
+
+// take a finished snapshot into account without blocking
+void poll_counters(bxg_map* m) {
+  if (m->ctr_pending && hipEventQuery(m->ev_ctr) == hipSuccess) {
+    m->ctr_pending = false;
+    m->known_leaves = std::min<uint64_t>(m->h_ctr->leaf_count, m->leaf_cap);
+    m->known_roots = m->h_ctr->root_count;
+  }
+}
+
+int grow_leaves(bxg_map* m, uint64_t need) {
+  bxg_ctx* c = m->ctx;
+  if (need <= m->leaf_cap) return BXG_OK;
+  uint64_t cap = std::max<uint64_t>(need, (uint64_t)m->leaf_cap * 2);
+  cap = (cap + 255) & ~255ull;
+  if (cap >= 0xFFFFFF00ull) return fail(c, BXG_ERR_OUT_OF_POOL, "leaf pool limit (2^32 leaves)");
+  const uint64_t more = (cap - m->leaf_cap) * leaf_bytes(m);
+  BXG_TRY(budget_check(c, more));
+  BXG_TRY(read_counters(m));  // also synchronizes the stream
+  const uint64_t used = std::min<uint64_t>(m->h_ctr->leaf_count, m->leaf_cap);
+  const uint64_t lw = m->h.leaf_words, cb = (uint64_t)m->h.leaf_size * m->h.value_bytes;
+  void *masks = nullptr, *cells = nullptr, *orig = nullptr;
+  BXG_TRY(dmalloc(c, &masks, cap * lw * 8));
+  if (cb) BXG_TRY(dmalloc(c, &cells, cap * cb));
+  BXG_TRY(dmalloc(c, &orig, cap * sizeof(int4)));
+  BXG_HIP(c, hipMemsetAsync(masks, 0, cap * lw * 8, c->stream));
+  if (cb) BXG_HIP(c, hipMemsetAsync(cells, 0, cap * cb, c->stream));
+  if (m->leaf_cap) {
+    BXG_HIP(c, hipMemcpyAsync(masks, m->h.leaf_masks, (uint64_t)m->leaf_cap * lw * 8,
+                              hipMemcpyDeviceToDevice, c->stream));
+    if (cb)
+      BXG_HIP(c, hipMemcpyAsync(cells, m->h.cells, (uint64_t)m->leaf_cap * cb,
+                                hipMemcpyDeviceToDevice, c->stream));
+    BXG_HIP(c, hipMemcpyAsync(orig, m->h.leaf_origin, (uint64_t)m->leaf_cap * sizeof(int4),
+                              hipMemcpyDeviceToDevice, c->stream));
+  }
+  BXG_TRY(ctx_sync(c));
+  dfree(m->h.leaf_masks);
+  dfree(m->h.cells);
+  dfree(m->h.leaf_origin);
+  m->h.leaf_masks = (uint64_t*)masks;
+  m->h.cells = (uint8_t*)cells;
+  m->h.leaf_origin = (int4*)orig;
+  m->h.leaf_cap = (uint32_t)cap;
+  c->used += more;
+  m->bytes += more;
+  m->leaf_cap = (uint32_t)cap;
+  // allocations that failed for lack of room ran the bump counter past the
+  // old capacity; every index below it is in use
+  uint32_t lc = (uint32_t)used;
+  BXG_HIP(c, hipMemcpyAsync(&m->ctr->leaf_count, &lc, 4, hipMemcpyHostToDevice, c->stream));
+  BXG_TRY(ctx_sync(c));
+  m->known_leaves = used;
+  return push_header(m);
+}
+
+int grow_roots(bxg_map* m, uint64_t need_roots) {
+  bxg_ctx* c = m->ctx;
+  uint64_t cap = pow2_at_least(std::max<uint64_t>(need_roots * 2, 1024));
+  if (cap <= m->root_cap) return BXG_OK;
+  if (cap > (1ull << 24)) return fail(c, BXG_ERR_OUT_OF_POOL, "root directory limit (2^23 roots)");
+  const uint64_t more = (cap - m->root_cap) * root_bytes(m);
+  BXG_TRY(budget_check(c, more));
+  BXG_TRY(ctx_sync(c));
+  void *keys = nullptr, *children = nullptr;
+  BXG_TRY(dmalloc(c, &keys, cap * 8));
+  BXG_TRY(dmalloc(c, &children, cap * m->h.inner_size * 4));
+  BXG_TRY(fill64(c, (uint64_t*)keys, cap, kEmptyKey));
+  BXG_TRY(fill32(c, (uint32_t*)children, cap * m->h.inner_size, kEmptyChild));
+  uint64_t* old_keys = m->h.root_keys;
+  uint32_t* old_children = m->h.children;
+  const uint32_t old_cap = m->root_cap;
+  m->h.root_keys = (uint64_t*)keys;
+  m->h.children = (uint32_t*)children;
+  m->h.root_mask = (uint32_t)(cap - 1);
+  m->h.max_probe = (uint32_t)std::min<uint64_t>(cap, 512);
+  m->root_cap = (uint32_t)cap;
+  BXG_TRY(push_header(m));
+  uint32_t zero = 0;
+  BXG_HIP(c, hipMemcpyAsync(&m->ctr->root_count, &zero, 4, hipMemcpyHostToDevice, c->stream));
+  if (old_keys) BXG_TRY(launch(c, "k_rehash", k_rehash, nblocks(old_cap), kBlock, (const MapDev*)m->d,
+                               (const uint64_t*)old_keys, (const uint32_t*)old_children, old_cap));
+  BXG_TRY(ctx_sync(c));
+  dfree(old_keys);
+  dfree(old_children);
+  c->used += more;
+  m->bytes += more;
+  return BXG_OK;
+}
+
+// keep head-room ahead of a call expected to add up to `leaves` leaves and
+// `roots` roots (estimates; the device reports an overflow if they are low)
+int ensure_room(bxg_map* m, uint64_t leaves, uint64_t roots) {
+  poll_counters(m);
+  const uint64_t want_leaves = m->known_leaves + leaves;
+  if (want_leaves > m->leaf_cap * 3 / 4) BXG_TRY(grow_leaves(m, want_leaves * 3 / 2 + 1024));
+  const uint64_t want_roots = m->known_roots + roots;
+  if (want_roots * 2 > m->root_cap) BXG_TRY(grow_roots(m, want_roots * 2));
+  return BXG_OK;
+}
+
+// ----------------------------------------------------------- scratch
+
+int ensure_scan_scratch(bxg_ctx* c, uint64_t points, uint32_t nscans) {
+  if (points > c->scan_cap) {
+    BXG_TRY(ctx_sync(c));
+    dfree(c->sc.rec);
+    dfree(c->sc.tpos);
+    dfree(c->sc.winners);
+    dfree(c->sc.ep_keys);
+    dfree(c->sc.ep_first);
+    c->sc = ScanScratch{};
+    const uint64_t cap = std::max<uint64_t>(pow2_at_least(points), 1 << 12);
+    if (cap > (1ull << 31)) return fail(c, BXG_ERR_INVALID_ARG, "too many points in one call");
+    const uint64_t tcap = cap * 2;
+    void* p;
+    BXG_TRY(dmalloc(c, &p, cap * sizeof(int4)));
+    c->sc.rec = (int4*)p;
+    BXG_TRY(dmalloc(c, &p, cap * 4));
+    c->sc.tpos = (uint32_t*)p;
+    BXG_TRY(dmalloc(c, &p, cap * 4));
+    c->sc.winners = (uint32_t*)p;
+    BXG_TRY(dmalloc(c, &p, tcap * 8));
+    c->sc.ep_keys = (uint64_t*)p;
+    BXG_TRY(dmalloc(c, &p, tcap * 4));
+    c->sc.ep_first = (uint32_t*)p;
+    c->sc.ep_mask = (uint32_t)(tcap - 1);
+    BXG_TRY(fill64(c, c->sc.ep_keys, tcap, kEmptyKey));
+    BXG_TRY(fill32(c, c->sc.ep_first, tcap, kNone));
+    c->scan_cap = cap;
+  }
+  c->sc.counters = c->d_counters;
+  if (nscans > c->desc_cap) {
+    BXG_TRY(ctx_sync(c));
+    dfree(c->d_desc);
+    c->d_desc = nullptr;
+    const uint32_t cap = (uint32_t)pow2_at_least(std::max<uint32_t>(nscans, 64));
+    void* p;
+    BXG_TRY(dmalloc(c, &p, (uint64_t)cap * sizeof(ScanDesc)));
+    c->d_desc = (ScanDesc*)p;
+    c->desc_cap = cap;
+  }
+  return BXG_OK;
+}
+
+int ensure_op_scratch(bxg_ctx* c, uint64_t n) {
+  if (n <= c->op_cap) return BXG_OK;
+  BXG_TRY(ctx_sync(c));
+  dfree(c->op_keys);
+  dfree(c->op_first);
+  dfree(c->op_last);
+  dfree(c->op_tpos);
+  dfree(c->op_prebit);
+  const uint64_t cap = std::max<uint64_t>(pow2_at_least(n), 1 << 12);
+  if (cap > (1ull << 31)) return fail(c, BXG_ERR_INVALID_ARG, "too many cells in one call");
+  const uint64_t tcap = cap * 2;
+  void* p;
+  BXG_TRY(dmalloc(c, &p, tcap * 8));
+  c->op_keys = (uint64_t*)p;
+  BXG_TRY(dmalloc(c, &p, tcap * 4));
+  c->op_first = (uint32_t*)p;
+  BXG_TRY(dmalloc(c, &p, tcap * 4));
+  c->op_last = (uint32_t*)p;
+  BXG_TRY(dmalloc(c, &p, cap * 4));
+  c->op_tpos = (uint32_t*)p;
+  BXG_TRY(dmalloc(c, &p, cap));
+  c->op_prebit = (uint8_t*)p;
+  c->op_mask = (uint32_t)(tcap - 1);
+  BXG_TRY(fill64(c, c->op_keys, tcap, kEmptyKey));
+  BXG_TRY(fill32(c, c->op_first, tcap, kNone));
+  BXG_TRY(fill32(c, c->op_last, tcap, 0));
+  c->op_cap = cap;
+  return BXG_OK;
+}
+
+OpScratch op_scratch(bxg_ctx* c) {
+  OpScratch os;
+  os.keys = c->op_keys;
+  os.first = c->op_first;
+  os.last = c->op_last;
+  os.mask = c->op_mask;
+  os.tpos = c->op_tpos;
+  os.prebit = c->op_prebit;
+  return os;
+}
+
+// ------------------------------------------------------------ scans
+
+bool is_prob(const bxg_map* m) { return (m->h.flags & kMapProbabilistic) != 0; }
+
+// posToCoord of the origin on the host, with the device's range rule
+int origin_voxel(bxg_map* m, const double o[3], int32_t out[3]) {
+  const double lim = (double)(1 << (20 + m->h.log2n));
+  for (int a = 0; a < 3; ++a) {
+    const double v = std::floor(o[a] * m->h.inv_res);
+    if (!(v >= -lim && v < lim))
+      return fail(m->ctx, BXG_ERR_COORD_RANGE, "origin outside the supported coordinate range");
+    out[a] = (int32_t)v;
+  }
+  return BXG_OK;
+}
+
+// Fill the device descriptors of one wave of scans (distinct maps).
+int build_descs(bxg_ctx* c, const bxg_scan_desc* scans, uint32_t nscans, uint32_t extra_flags,
+                std::vector<ScanDesc>& out, uint64_t* total) {
+  out.resize(nscans);
+  uint64_t off = 0;
+  for (uint32_t s = 0; s < nscans; ++s) {
+    const bxg_scan_desc& in = scans[s];
+    bxg_map* m = in.map;
+    if (!m || m->ctx != c) return fail(c, BXG_ERR_INVALID_ARG, "scan without a map of this context");
+    if (!is_prob(m)) return fail(c, BXG_ERR_INVALID_ARG, "insertPointCloud needs a probabilistic map");
+    if (in.n && !in.points_dev) return fail(c, BXG_ERR_INVALID_ARG, "null points");
+    if (std::isnan(in.max_range)) return fail(c, BXG_ERR_INVALID_ARG, "max_range is NaN");
+    ScanDesc& d = out[s];
+    std::memset(&d, 0, sizeof(d));
+    d.map = m->d;
+    d.points = in.points_dev;
+    d.pt_off = off;
+    d.n = (uint32_t)in.n;
+    d.flags = ((in.flags & BXG_POINTS_F64) ? kScanF64 : 0u) | extra_flags;
+    d.fx = in.origin[0];
+    d.fy = in.origin[1];
+    d.fz = in.origin[2];
+    d.max_range = in.max_range;
+    d.r2 = in.max_range * in.max_range;  // probabilistic_map.hpp:145
+    int32_t o[3];
+    BXG_TRY(origin_voxel(m, in.origin, o));
+    d.ox = o[0];
+    d.oy = o[1];
+    d.oz = o[2];
+    d.tag = s;
+    if (!(extra_flags & kScanNoRays) && !m->extra.empty()) {
+      d.extra = m->d_extra;
+      d.nextra = (uint32_t)m->extra.size();
+    }
+    if (in.n > 0xFFFFFFFFull) return fail(c, BXG_ERR_INVALID_ARG, "scan too large");
+    off += in.n;
+  }
+  *total = off;
+  if (off >= (1ull << 31)) return fail(c, BXG_ERR_INVALID_ARG, "too many points in one call");
+  return BXG_OK;
+}
+
+// upload pending addHit/MissPoint rays of a map to the device
+int upload_extra(bxg_map* m) {
+  bxg_ctx* c = m->ctx;
+  if (m->extra.empty()) return BXG_OK;
+  if (m->extra.size() > m->extra_cap) {
+    BXG_TRY(ctx_sync(c));
+    dfree(m->d_extra);
+    m->d_extra = nullptr;
+    const uint32_t cap = (uint32_t)pow2_at_least(std::max<uint64_t>(m->extra.size(), 64));
+    void* p;
+    BXG_TRY(dmalloc(c, &p, (uint64_t)cap * sizeof(int4)));
+    m->d_extra = (int4*)p;
+    m->extra_cap = cap;
+  }
+  return upload(c, m->d_extra, m->extra.data(), m->extra.size() * sizeof(int4));
+}
+
+// the kernel sequence of one wave of scans on given scratch/descriptors
+int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, ScanScratch sc,
+                         uint64_t grid_points, uint32_t nscans, bool any_extra) {
+  const uint32_t gp = nblocks(grid_points);
+  const uint32_t gw = std::max<uint32_t>(1, std::min<uint32_t>(gp, 16384));
+  BXG_TRY(launch(c, "k_scan_classify", k_scan_classify, gp, kBlock, hdr, desc, sc));
+  BXG_TRY(launch(c, "k_scan_winners", k_scan_winners, gp, kBlock, hdr, desc, sc));
+  if (any_extra)
+    BXG_TRY(launch(c, "k_scan_extra_leaves", k_scan_extra_leaves, nscans, kBlock, hdr, desc));
+  BXG_TRY(launch(c, "k_scan_endpoints", k_scan_endpoints, gw, kBlock, desc, sc));
+  if (c->counting) {
+    BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<true>, gw, kBlock, desc, sc));
+    if (any_extra)
+      BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<true>, nscans, kBlock, hdr, desc));
+  } else {
+    BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<false>, gw, kBlock, desc, sc));
+    if (any_extra)
+      BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<false>, nscans, kBlock, hdr, desc));
+  }
+  BXG_TRY(launch(c, "k_scan_finalize", k_scan_finalize, 1, kBlock, hdr, desc, sc));
+  return BXG_OK;
+}
+
+// reset the per-scan statistics of the maps of a counted run
+int reset_scan_stats(bxg_map* m) {
+  bxg_ctx* c = m->ctx;
+  const uint64_t z[2] = {0, 0};
+  return upload(c, &m->ctr->stat_rays, z, sizeof(z));
+}
+
+// one wave of scans, each into a different map
+int run_wave(bxg_ctx* c, const bxg_scan_desc* scans, uint32_t nscans, uint32_t extra_flags) {
+  std::vector<ScanDesc> descs;
+  uint64_t total = 0;
+  BXG_TRY(build_descs(c, scans, nscans, extra_flags, descs, &total));
+  bool any_extra = false;
+  for (uint32_t s = 0; s < nscans; ++s) {
+    bxg_map* m = scans[s].map;
+    // head-room for a scan: about one new leaf per point covers a first
+    // LiDAR scan into an empty map (long free-space rays); an overflow is
+    // still caught on the device and leaves the cells untouched
+    BXG_TRY(ensure_room(m, std::min<uint64_t>(scans[s].n + 4096, 1u << 24), 1024));
+    if (descs[s].nextra) {
+      any_extra = true;
+      BXG_TRY(upload_extra(m));
+      descs[s].extra = m->d_extra;
+    }
+    if (c->counting) BXG_TRY(reset_scan_stats(m));
+  }
+  // the maps may have grown: descriptors point at device headers, which
+  // stay put, so nothing to rebuild
+  BXG_TRY(ensure_scan_scratch(c, std::max<uint64_t>(total, 1), nscans));
+  BatchHdr hdr{nscans, 0, total};
+  BXG_TRY(upload(c, c->d_desc, descs.data(), descs.size() * sizeof(ScanDesc)));
+  BXG_TRY(upload(c, c->d_hdr, &hdr, sizeof(hdr)));
+  BXG_TRY(enqueue_scan_kernels(c, c->d_hdr, c->d_desc, c->sc, total, nscans, any_extra));
+  for (uint32_t s = 0; s < nscans; ++s) BXG_TRY(snapshot_counters(scans[s].map));
+  return BXG_OK;
+}
+
+// split a batch into waves of distinct maps, keeping input order per map
+int run_batch(bxg_ctx* c, const bxg_scan_desc* scans, uint32_t nscans, uint32_t extra_flags) {
+  uint32_t start = 0;
+  while (start < nscans) {
+    uint32_t end = start;
+    std::vector<bxg_map*> seen;
+    while (end < nscans && std::find(seen.begin(), seen.end(), scans[end].map) == seen.end()) {
+      seen.push_back(scans[end].map);
+      ++end;
+    }
+    BXG_TRY(run_wave(c, scans + start, end - start, extra_flags));
+    start = end;
+  }
+  return BXG_OK;
+}
+
+int status_code(uint32_t st) {
+  if (!st) return BXG_OK;
+  if (st & kStCoordRange) return BXG_ERR_COORD_RANGE;
+  if (st & (kStOutOfPool | kStRootFull)) return BXG_ERR_OUT_OF_POOL;
+  return BXG_ERR_INVALID_ARG;
+}
+
+int map_status(bxg_map* m) {
+  BXG_TRY(read_counters(m));
+  const uint32_t st = m->h_ctr->status;
+  if (st) {
+    const int code = status_code(st);
+    return fail(m->ctx, code, std::string("device error on map: ") + bxg_status_string(code));
+  }
+  return BXG_OK;
+}
+
+int clear_status(bxg_map* m) {
+  const uint32_t z = 0;
+  BXG_HIP(m->ctx, hipMemcpyAsync(&m->ctr->status, &z, 4, hipMemcpyHostToDevice, m->ctx->stream));
+  BXG_TRY(ctx_sync(m->ctx));
+  // any snapshot enqueued before has landed; the mirror follows the device
+  m->ctr_pending = false;
+  m->h_ctr->status = 0;
+  return BXG_OK;
+}
+
+// grow after a device overflow, from the counters read at the failure
+int grow_after_overflow(bxg_map* m) {
+  BXG_TRY(read_counters(m));
+  const uint32_t st = m->h_ctr->status;
+  BXG_TRY(clear_status(m));
+  if (st & kStOutOfPool) BXG_TRY(grow_leaves(m, (uint64_t)m->leaf_cap * 2));
+  if (st & kStRootFull) BXG_TRY(grow_roots(m, (uint64_t)m->root_cap));
+  return BXG_OK;
+}
+
+// ---------------------------------------------------------- bulk ops
+
+int check_coords_args(bxg_map* m, const void* coords, uint64_t n) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  if (n && !coords) return fail(m->ctx, BXG_ERR_INVALID_ARG, "null coordinates");
+  if (n >= (1ull << 31)) return fail(m->ctx, BXG_ERR_INVALID_ARG, "too many coordinates in one call");
+  return BXG_OK;
+}
+
+int map_failed(bxg_map* m) {
+  poll_counters(m);
+  if (!m->ctr_pending && m->h_ctr->status)
+    return fail(m->ctx, BXG_ERR_MAP_FAILED, "the map holds a device error; call bxg_map_clear_error");
+  return BXG_OK;
+}
+
+int cell_op(bxg_map* m, const int32_t* coords, const void* vals, uint64_t n, uint8_t* was_on,
+            uint32_t op) {
+  bxg_ctx* c = m->ctx;
+  BXG_TRY(check_coords_args(m, coords, n));
+  if (!n) return BXG_OK;
+  if (op != kOpSetOff) {
+    BXG_TRY(ensure_room(m, std::min<uint64_t>(n, 1u << 22), std::min<uint64_t>(n, 1u << 16)));
+    BXG_TRY(launch(c, "k_touch", k_touch, nblocks(n), kBlock, (const MapDev*)m->d, coords, n));
+  }
+  BXG_TRY(ensure_op_scratch(c, n));
+  const OpScratch os = op_scratch(c);
+  BXG_TRY(launch(c, "k_op_resolve", k_op_resolve, nblocks(n), kBlock, (const MapDev*)m->d, coords,
+                 n, os));
+  BXG_TRY(launch(c, "k_op_apply", k_op_apply, nblocks(n), kBlock, (const MapDev*)m->d, coords,
+                 (const uint8_t*)vals, n, op, was_on, os));
+  BXG_TRY(launch(c, "k_op_release", k_op_release, nblocks(n), kBlock, n, os));
+  return snapshot_counters(m);
+}
+
+int known_leaf_count(bxg_map* m, uint32_t* out) {
+  BXG_TRY(read_counters(m));
+  *out = (uint32_t)m->known_leaves;
+  return BXG_OK;
+}
+
+}  // namespace
+
+// ====================================================================
+//                                C ABI
+// ====================================================================
+
+extern "C" {
+
+const char* bxg_status_string(int status) {
+  switch (status) {
+    case BXG_OK: return "ok";
+    case BXG_ERR_INVALID_ARG: return "invalid argument";
+    case BXG_ERR_OUT_OF_POOL: return "node pool or root directory exhausted";
+    case BXG_ERR_HIP: return "HIP runtime error";
+    case BXG_ERR_COORD_RANGE: return "coordinate outside the supported range";
+    case BXG_ERR_BUDGET: return "memory budget exceeded";
+    case BXG_ERR_NOT_SUPPORTED: return "not supported";
+    case BXG_ERR_CAPACITY: return "output buffer too small";
+    case BXG_ERR_MAP_FAILED: return "map holds an unacknowledged device error";
+    default: return "unknown status";
+  }
+}
+
+int bxg_ctx_create(int device, uint64_t budget_bytes, bxg_ctx** out) {
+  if (!out) return BXG_ERR_INVALID_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return BXG_ERR_HIP;
+  if (device < 0 || device >= ndev) return BXG_ERR_INVALID_ARG;
+  if (hipSetDevice(device) != hipSuccess) return BXG_ERR_HIP;
+  bxg_ctx* c = new bxg_ctx();
+  c->device = device;
+  c->budget = budget_bytes;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    c->cu_count = prop.multiProcessorCount;
+    c->hbm_bytes = prop.totalGlobalMem;
+    c->name = prop.gcnArchName;
+  }
+  int rc = BXG_OK;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = BXG_ERR_HIP;
+  void* p = nullptr;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, 64);
+  c->d_counters = (uint32_t*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, 64);
+  c->d_hdr = (BatchHdr*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, 64);
+  c->d_small = (uint64_t*)p;
+  if (rc == BXG_OK && hipHostMalloc((void**)&c->h_small, 64, hipHostMallocDefault) != hipSuccess)
+    rc = BXG_ERR_HIP;
+  for (int i = 0; rc == BXG_OK && i < 64; ++i)
+    if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = BXG_ERR_HIP;
+  if (rc == BXG_OK) rc = fill32(c, c->d_counters, 16, 0);
+  if (rc == BXG_OK) rc = ctx_sync(c);
+  if (rc != BXG_OK) {
+    bxg_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return BXG_OK;
+}
+
+int bxg_ctx_destroy(bxg_ctx* c) {
+  if (!c) return BXG_ERR_INVALID_ARG;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  while (!c->maps.empty()) bxg_map_destroy(c->maps.back());
+  dfree(c->sc.rec);
+  dfree(c->sc.tpos);
+  dfree(c->sc.winners);
+  dfree(c->sc.ep_keys);
+  dfree(c->sc.ep_first);
+  dfree(c->d_counters);
+  dfree(c->d_desc);
+  dfree(c->d_hdr);
+  dfree(c->op_keys);
+  dfree(c->op_first);
+  dfree(c->op_last);
+  dfree(c->op_tpos);
+  dfree(c->op_prebit);
+  for (int i = 0; i < 4; ++i) dfree(c->dstage[i]);
+  dfree(c->d_small);
+  if (c->h_small) hipHostFree(c->h_small);
+  for (auto& s : c->stage) {
+    if (s.host) hipHostFree(s.host);
+    if (s.done) hipEventDestroy(s.done);
+  }
+  for (auto& e : c->ev)
+    if (e) hipEventDestroy(e);
+  for (auto& p : c->prof_pending) {
+    hipEventDestroy(p.a);
+    hipEventDestroy(p.b);
+  }
+  for (auto e : c->ev_free) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return BXG_OK;
+}
+
+int bxg_ctx_sync(bxg_ctx* c) {
+  if (!c) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  return ctx_sync(c);
+}
+
+const char* bxg_ctx_last_error(bxg_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int bxg_ctx_stream(bxg_ctx* c, void** s) {
+  if (!c || !s) return BXG_ERR_INVALID_ARG;
+  *s = (void*)c->stream;
+  return BXG_OK;
+}
+
+int bxg_ctx_memory(bxg_ctx* c, uint64_t* used, uint64_t* budget) {
+  if (!c) return BXG_ERR_INVALID_ARG;
+  if (used) *used = c->used;
+  if (budget) *budget = c->budget;
+  return BXG_OK;
+}
+
+int bxg_ctx_device_info(bxg_ctx* c, char* name, uint64_t cap, int* cu, uint64_t* hbm) {
+  if (!c) return BXG_ERR_INVALID_ARG;
+  if (name && cap) {
+    std::strncpy(name, c->name.c_str(), cap - 1);
+    name[cap - 1] = 0;
+  }
+  if (cu) *cu = c->cu_count;
+  if (hbm) *hbm = c->hbm_bytes;
+  return BXG_OK;
+}
+
+int bxg_malloc(bxg_ctx* c, uint64_t bytes, void** out) {
+  if (!c || !out) return BXG_ERR_INVALID_ARG;
+  return dmalloc(c, out, bytes);
+}
+int bxg_free(bxg_ctx* c, void* p) {
+  if (!c) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(ctx_sync(c));
+  dfree(p);
+  return BXG_OK;
+}
+int bxg_memcpy_h2d(bxg_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  return h2d_sync(c, dst, src, bytes);
+}
+int bxg_memcpy_d2h(bxg_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  return d2h_sync(c, dst, src, bytes);
+}
+int bxg_memcpy_d2d(bxg_ctx* c, void* dst, const void* src, uint64_t bytes) {
+  if (!c) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (!bytes) return BXG_OK;
+  BXG_HIP(c, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return BXG_OK;
+}
+
+int bxg_event_record(bxg_ctx* c, uint32_t slot) {
+  if (!c || slot >= 64) return BXG_ERR_INVALID_ARG;
+  BXG_HIP(c, hipEventRecord(c->ev[slot], c->stream));
+  return BXG_OK;
+}
+int bxg_event_elapsed(bxg_ctx* c, uint32_t a, uint32_t b, float* ms) {
+  if (!c || a >= 64 || b >= 64 || !ms) return BXG_ERR_INVALID_ARG;
+  BXG_HIP(c, hipEventSynchronize(c->ev[b]));
+  BXG_HIP(c, hipEventElapsedTime(ms, c->ev[a], c->ev[b]));
+  return BXG_OK;
+}
+
+static int prof_fold(bxg_ctx* c) {
+  if (c->prof_pending.empty()) return BXG_OK;
+  BXG_TRY(ctx_sync(c));
+  for (auto& p : c->prof_pending) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, p.a, p.b);
+    auto& t = c->prof_totals[p.name];
+    t.first += 1;
+    t.second += ms;
+    c->ev_free.push_back(p.a);
+    c->ev_free.push_back(p.b);
+  }
+  c->prof_pending.clear();
+  return BXG_OK;
+}
+
+int bxg_profile_enable(bxg_ctx* c, int on) {
+  if (!c) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(prof_fold(c));
+  c->profiling = on != 0;
+  return BXG_OK;
+}
+int bxg_profile_read(bxg_ctx* c, const char* kernel, uint64_t* launches, double* total_ms) {
+  if (!c || !kernel) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(prof_fold(c));
+  auto it = c->prof_totals.find(kernel);
+  if (launches) *launches = it == c->prof_totals.end() ? 0 : it->second.first;
+  if (total_ms) *total_ms = it == c->prof_totals.end() ? 0.0 : it->second.second;
+  return BXG_OK;
+}
+int bxg_profile_reset(bxg_ctx* c) {
+  if (!c) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(prof_fold(c));
+  c->prof_totals.clear();
+  return BXG_OK;
+}
+int bxg_set_counting(bxg_ctx* c, int on) {
+  if (!c) return BXG_ERR_INVALID_ARG;
+  c->counting = on != 0;
+  return BXG_OK;
+}
+
+// -------------------------------------------------------------- maps
+
+int bxg_map_create(bxg_ctx* c, double voxel_size, uint32_t inner_bits, uint32_t leaf_bits,
+                   uint32_t value_bytes, uint32_t flags, bxg_map** out) {
+  if (!c || !out) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  *out = nullptr;
+  // VoxelGrid's constructor rejects bits < 1 (bonxai.hpp:399-401)
+  if (inner_bits < 1 || leaf_bits < 1)
+    return fail(c, BXG_ERR_INVALID_ARG, "The minimum value of the inner_bits and leaf_bits should be 1");
+  if (inner_bits > 4 || leaf_bits > 4)
+    return fail(c, BXG_ERR_NOT_SUPPORTED, "inner_bits and leaf_bits above 4 are not supported");
+  if (!(voxel_size > 0.0) || !std::isfinite(voxel_size))
+    return fail(c, BXG_ERR_INVALID_ARG, "voxel_size must be positive and finite");
+  if (value_bytes > 256) return fail(c, BXG_ERR_NOT_SUPPORTED, "cell values above 256 bytes");
+  if ((flags & BXG_MAP_PROBABILISTIC) && value_bytes != 4)
+    return fail(c, BXG_ERR_INVALID_ARG, "probabilistic cells are 4 bytes");
+  bxg_map* m = new bxg_map();
+  m->ctx = c;
+  MapDev& h = m->h;
+  h.res = voxel_size;
+  h.inv_res = 1.0 / voxel_size;  // bonxai.hpp:395
+  h.inner_bits = inner_bits;
+  h.leaf_bits = leaf_bits;
+  h.log2n = inner_bits + leaf_bits;
+  h.inner_size = 1u << (3 * inner_bits);
+  h.leaf_size = 1u << (3 * leaf_bits);
+  h.leaf_words = h.leaf_size >= 64 ? h.leaf_size / 64 : 1;
+  h.value_bytes = value_bytes;
+  h.flags = (flags & BXG_MAP_PROBABILISTIC) ? kMapProbabilistic : 0u;
+  // Options defaults (probabilistic_map.hpp:56-64): logods() of 0.4f, 0.7f,
+  // 0.12f, 0.97f and 0.5
+  h.opt_miss = -405465;
+  h.opt_hit = 847297;
+  h.opt_cmin = -1992430;
+  h.opt_cmax = 3476099;
+  h.opt_thr = 0;
+  int rc = BXG_OK;
+  void* p = nullptr;
+  rc = dmalloc(c, &p, sizeof(MapDev));
+  m->d = (MapDev*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, sizeof(MapCounters));
+  m->ctr = (MapCounters*)p;
+  if (rc == BXG_OK && hipHostMalloc((void**)&m->h_ctr, sizeof(MapCounters), hipHostMallocDefault) != hipSuccess)
+    rc = BXG_ERR_HIP;
+  if (rc == BXG_OK && hipEventCreateWithFlags(&m->ev_ctr, hipEventDisableTiming) != hipSuccess)
+    rc = BXG_ERR_HIP;
+  if (rc == BXG_OK) {
+    std::memset(m->h_ctr, 0, sizeof(MapCounters));
+    m->h_ctr->cur = 1;  // _update_count starts at 1 (probabilistic_map.hpp:129)
+    h.ctr = m->ctr;
+    rc = h2d_sync(c, m->ctr, m->h_ctr, sizeof(MapCounters));
+  }
+  c->maps.push_back(m);
+  if (rc == BXG_OK) rc = grow_roots(m, 2048);
+  if (rc == BXG_OK) rc = grow_leaves(m, 4096);
+  if (rc == BXG_OK) rc = push_header(m);
+  if (rc == BXG_OK) rc = ctx_sync(c);
+  if (rc != BXG_OK) {
+    bxg_map_destroy(m);
+    return rc;
+  }
+  *out = m;
+  return BXG_OK;
+}
+
+int bxg_map_destroy(bxg_map* m) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  hipStreamSynchronize(c->stream);
+  dfree(m->h.root_keys);
+  dfree(m->h.children);
+  dfree(m->h.leaf_masks);
+  dfree(m->h.cells);
+  dfree(m->h.leaf_origin);
+  dfree(m->d);
+  dfree(m->ctr);
+  dfree(m->d_extra);
+  if (m->h_ctr) hipHostFree(m->h_ctr);
+  if (m->ev_ctr) hipEventDestroy(m->ev_ctr);
+  c->used -= std::min(c->used, m->bytes);
+  c->maps.erase(std::remove(c->maps.begin(), c->maps.end(), m), c->maps.end());
+  delete m;
+  return BXG_OK;
+}
+
+int bxg_map_reserve(bxg_map* m, uint64_t leaves, uint64_t roots) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  if (leaves > m->leaf_cap) BXG_TRY(grow_leaves(m, leaves));
+  if (roots * 2 > m->root_cap) BXG_TRY(grow_roots(m, roots));
+  return BXG_OK;
+}
+
+int bxg_map_clear(bxg_map* m, int option) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  uint32_t nl = 0;
+  BXG_TRY(known_leaf_count(m, &nl));
+  if (option == BXG_SET_ALL_CELLS_OFF) {
+    const uint64_t work = std::max<uint64_t>((uint64_t)nl * m->h.leaf_words,
+                                             (uint64_t)nl * m->h.leaf_size * m->h.value_bytes / 4);
+    BXG_TRY(launch(c, "k_cells_off", k_cells_off, (uint32_t)std::min<uint64_t>(nblocks(work), 8192),
+                   kBlock, (const MapDev*)m->d, nl));
+    return ctx_sync(c);
+  }
+  if (option != BXG_CLEAR_MEMORY) return fail(c, BXG_ERR_INVALID_ARG, "unknown clear option");
+  // CLEAR_MEMORY: empty directory and pool (the pools keep their capacity)
+  BXG_TRY(fill64(c, m->h.root_keys, m->root_cap, kEmptyKey));
+  BXG_TRY(fill32(c, m->h.children, (uint64_t)m->root_cap * m->h.inner_size, kEmptyChild));
+  if (nl) {
+    BXG_HIP(c, hipMemsetAsync(m->h.leaf_masks, 0, (uint64_t)nl * m->h.leaf_words * 8, c->stream));
+    if (m->h.value_bytes)
+      BXG_HIP(c, hipMemsetAsync(m->h.cells, 0, (uint64_t)nl * m->h.leaf_size * m->h.value_bytes,
+                                c->stream));
+  }
+  const uint32_t z[3] = {0, 0, 0};  // leaf_count, root_count, status
+  BXG_HIP(c, hipMemcpyAsync(m->ctr, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+  m->extra.clear();
+  BXG_TRY(ctx_sync(c));
+  m->known_leaves = m->known_roots = 0;
+  return BXG_OK;
+}
+
+int bxg_map_sync(bxg_map* m) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  return map_status(m);
+}
+
+int bxg_map_clear_error(bxg_map* m) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(read_counters(m));
+  const uint32_t st = m->h_ctr->status;
+  BXG_TRY(clear_status(m));
+  // make room for what failed
+  if (st & kStOutOfPool) BXG_TRY(grow_leaves(m, (uint64_t)m->leaf_cap * 2));
+  if (st & kStRootFull) BXG_TRY(grow_roots(m, (uint64_t)m->root_cap));
+  return BXG_OK;
+}
+
+int bxg_map_get_stats(bxg_map* m, bxg_map_stats* out) {
+  if (!m || !out) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(read_counters(m));
+  const MapCounters& k = *m->h_ctr;
+  out->leaf_count = std::min<uint64_t>(k.leaf_count, m->leaf_cap);
+  out->leaf_capacity = m->leaf_cap;
+  out->root_count = k.root_count;
+  out->root_capacity = m->root_cap;
+  out->scans_applied = k.scans_applied;
+  out->device_bytes = m->bytes;
+  out->last_rays = k.stat_rays;
+  out->last_ray_updates = k.stat_ray_updates;
+  out->update_count = (uint32_t)k.cur;
+  out->status = k.status;
+  return BXG_OK;
+}
+
+int bxg_map_active_cells(bxg_map* m, uint64_t* out) {
+  if (!m || !out) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  uint32_t nl = 0;
+  BXG_TRY(known_leaf_count(m, &nl));
+  BXG_HIP(c, hipMemsetAsync(c->d_small, 0, 8, c->stream));
+  BXG_TRY(launch(c, "k_count_on", k_count_on, nblocks((uint64_t)nl * m->h.leaf_words), kBlock,
+                 (const MapDev*)m->d, nl, c->d_small));
+  BXG_TRY(d2h_sync(c, c->h_small, c->d_small, 8));
+  *out = c->h_small[0];
+  return BXG_OK;
+}
+
+int bxg_map_mem_usage(bxg_map* m, uint64_t* out) {
+  if (!m || !out) return BXG_ERR_INVALID_ARG;
+  *out = m->bytes + sizeof(MapDev) + sizeof(MapCounters);
+  return BXG_OK;
+}
+
+double bxg_map_voxel_size(bxg_map* m) { return m ? m->h.res : 0.0; }
+
+int bxg_map_digest(bxg_map* m, uint64_t out[4]) {
+  if (!m || !out) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  uint32_t nl = 0;
+  BXG_TRY(known_leaf_count(m, &nl));
+  BXG_HIP(c, hipMemsetAsync(c->d_small, 0, 32, c->stream));
+  BXG_TRY(launch(c, "k_digest", k_digest, nblocks((uint64_t)nl * m->h.leaf_size), kBlock,
+                 (const MapDev*)m->d, nl, c->d_small));
+  BXG_TRY(d2h_sync(c, c->h_small, c->d_small, 32));
+  std::memcpy(out, c->h_small, 32);
+  return BXG_OK;
+}
+
+int bxg_map_release_unused(bxg_map* m) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  return fail(m->ctx, BXG_ERR_NOT_SUPPORTED, "releaseUnusedMemory is not implemented yet");
+}
+
+// ------------------------------------------------------ bulk VoxelGrid
+
+int bxg_set_values(bxg_map* m, const int32_t* coords, const void* vals, uint64_t n, uint8_t* was_on) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  if (m->h.value_bytes && n && !vals) return fail(m->ctx, BXG_ERR_INVALID_ARG, "null values");
+  return cell_op(m, coords, vals, n, was_on, kOpSetValue);
+}
+int bxg_set_cells_on(bxg_map* m, const int32_t* coords, const void* defaults, uint64_t n,
+                     uint8_t* was_on) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  return cell_op(m, coords, defaults, n, was_on, kOpSetOn);
+}
+int bxg_set_cells_off(bxg_map* m, const int32_t* coords, uint64_t n, uint8_t* was_on) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  return cell_op(m, coords, nullptr, n, was_on, kOpSetOff);
+}
+int bxg_lookup(bxg_map* m, const int32_t* coords, uint64_t n, void* vals, uint8_t* found) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(check_coords_args(m, coords, n));
+  if (!n) return BXG_OK;
+  return launch(m->ctx, "k_lookup", k_lookup, nblocks(n), kBlock, (const MapDev*)m->d, coords, n,
+                (uint8_t*)vals, found);
+}
+int bxg_classify(bxg_map* m, const int32_t* coords, uint64_t n, int8_t* state) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(check_coords_args(m, coords, n));
+  if (!is_prob(m)) return fail(m->ctx, BXG_ERR_INVALID_ARG, "classify needs a probabilistic map");
+  if (n && !state) return fail(m->ctx, BXG_ERR_INVALID_ARG, "null state buffer");
+  if (!n) return BXG_OK;
+  return launch(m->ctx, "k_classify_cells", k_classify_cells, nblocks(n), kBlock,
+                (const MapDev*)m->d, coords, n, state);
+}
+
+int bxg_export(bxg_map* m, int filter, int32_t* coords, void* vals, uint64_t cap, uint64_t* count) {
+  if (!m || !count) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (filter < 0 || filter > 2) return fail(c, BXG_ERR_INVALID_ARG, "unknown export filter");
+  if (filter && !is_prob(m)) return fail(c, BXG_ERR_INVALID_ARG, "filtered export needs a probabilistic map");
+  uint32_t nl = 0;
+  BXG_TRY(known_leaf_count(m, &nl));
+  BXG_HIP(c, hipMemsetAsync(c->d_small, 0, 8, c->stream));
+  BXG_TRY(launch(c, "k_export", k_export, nblocks((uint64_t)nl * m->h.leaf_size), kBlock,
+                 (const MapDev*)m->d, nl, (uint32_t)filter, coords, (uint8_t*)vals, cap,
+                 (uint32_t*)c->d_small));
+  BXG_TRY(d2h_sync(c, c->h_small, c->d_small, 8));
+  *count = (uint32_t)c->h_small[0];
+  return *count > cap ? fail(c, BXG_ERR_CAPACITY, "export buffer too small") : BXG_OK;
+}
+
+// host forms ---------------------------------------------------------
+
+static int stage_in(bxg_ctx* c, int slot, const void* src, uint64_t bytes, void** dev) {
+  BXG_TRY(dstage(c, slot, std::max<uint64_t>(bytes, 64), dev));
+  if (bytes && src) BXG_HIP(c, hipMemcpyAsync(*dev, src, bytes, hipMemcpyHostToDevice, c->stream));
+  return BXG_OK;
+}
+
+static int host_cell_op(bxg_map* m, const int32_t* coords, const void* vals, uint64_t n,
+                        uint8_t* was_on, uint32_t op) {
+  bxg_ctx* c = m->ctx;
+  BXG_TRY(check_coords_args(m, coords, n));
+  if (!n) return BXG_OK;
+  BXG_TRY(map_failed(m));
+  const uint32_t vb = m->h.value_bytes;
+  void *dc, *dv = nullptr, *dw = nullptr;
+  BXG_TRY(stage_in(c, 0, coords, n * 12, &dc));
+  if (vals && vb) BXG_TRY(stage_in(c, 1, vals, n * vb, &dv));
+  if (was_on) BXG_TRY(stage_in(c, 2, nullptr, n, &dw));
+  for (int attempt = 0;; ++attempt) {
+    BXG_TRY(cell_op(m, (const int32_t*)dc, dv, n, (uint8_t*)dw, op));
+    BXG_TRY(read_counters(m));
+    if (!m->h_ctr->status) break;
+    if (m->h_ctr->status & kStCoordRange) {
+      BXG_TRY(clear_status(m));
+      return fail(c, BXG_ERR_COORD_RANGE, "coordinate outside the supported range");
+    }
+    if (attempt > 8) return fail(c, BXG_ERR_OUT_OF_POOL, "could not grow the node pool");
+    BXG_TRY(grow_after_overflow(m));
+  }
+  if (was_on) BXG_TRY(d2h_sync(c, was_on, dw, n));
+  return BXG_OK;
+}
+
+int bxg_set_values_host(bxg_map* m, const int32_t* coords, const void* vals, uint64_t n,
+                        uint8_t* was_on) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  if (m->h.value_bytes && n && !vals) return fail(m->ctx, BXG_ERR_INVALID_ARG, "null values");
+  return host_cell_op(m, coords, vals, n, was_on, kOpSetValue);
+}
+int bxg_set_cells_on_host(bxg_map* m, const int32_t* coords, const void* defaults, uint64_t n,
+                          uint8_t* was_on) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  return host_cell_op(m, coords, defaults, n, was_on, kOpSetOn);
+}
+int bxg_set_cells_off_host(bxg_map* m, const int32_t* coords, uint64_t n, uint8_t* was_on) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  return host_cell_op(m, coords, nullptr, n, was_on, kOpSetOff);
+}
+int bxg_lookup_host(bxg_map* m, const int32_t* coords, uint64_t n, void* vals, uint8_t* found) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(check_coords_args(m, coords, n));
+  if (!n) return BXG_OK;
+  const uint32_t vb = m->h.value_bytes;
+  void *dc, *dv = nullptr, *df = nullptr;
+  BXG_TRY(stage_in(c, 0, coords, n * 12, &dc));
+  if (vals && vb) BXG_TRY(stage_in(c, 1, nullptr, n * vb, &dv));
+  if (found) BXG_TRY(stage_in(c, 2, nullptr, n, &df));
+  BXG_TRY(launch(c, "k_lookup", k_lookup, nblocks(n), kBlock, (const MapDev*)m->d,
+                 (const int32_t*)dc, n, (uint8_t*)dv, (uint8_t*)df));
+  if (dv) BXG_HIP(c, hipMemcpyAsync(vals, dv, n * vb, hipMemcpyDeviceToHost, c->stream));
+  if (df) BXG_HIP(c, hipMemcpyAsync(found, df, n, hipMemcpyDeviceToHost, c->stream));
+  return ctx_sync(c);
+}
+int bxg_classify_host(bxg_map* m, const int32_t* coords, uint64_t n, int8_t* state) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(check_coords_args(m, coords, n));
+  if (!n) return BXG_OK;
+  void *dc, *ds;
+  BXG_TRY(stage_in(c, 0, coords, n * 12, &dc));
+  BXG_TRY(stage_in(c, 2, nullptr, n, &ds));
+  BXG_TRY(bxg_classify(m, (const int32_t*)dc, n, (int8_t*)ds));
+  return d2h_sync(c, state, ds, n);
+}
+int bxg_export_host(bxg_map* m, int filter, int32_t* coords, void* vals, uint64_t cap,
+                    uint64_t* count) {
+  if (!m || !count) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  const uint32_t vb = m->h.value_bytes;
+  void *dc = nullptr, *dv = nullptr;
+  if (coords && cap) BXG_TRY(dstage(c, 0, cap * 12, &dc));
+  if (vals && vb && cap) BXG_TRY(dstage(c, 1, cap * vb, &dv));
+  const int rc = bxg_export(m, filter, (int32_t*)dc, dv, cap, count);
+  if (rc != BXG_OK && rc != BXG_ERR_CAPACITY) return rc;
+  const uint64_t k = std::min(*count, cap);
+  if (dc && k) BXG_HIP(c, hipMemcpyAsync(coords, dc, k * 12, hipMemcpyDeviceToHost, c->stream));
+  if (dv && k) BXG_HIP(c, hipMemcpyAsync(vals, dv, k * vb, hipMemcpyDeviceToHost, c->stream));
+  BXG_TRY(ctx_sync(c));
+  return rc;
+}
+
+int bxg_read_leaves_host(bxg_map* m, const int32_t* origins, uint64_t n, uint64_t* masks,
+                         void* cells, uint8_t* exists) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (!n) return BXG_OK;
+  if (!origins || !masks || !exists || (m->h.value_bytes && !cells) || n >= (1u << 31))
+    return fail(c, BXG_ERR_INVALID_ARG, "bad leaf read arguments");
+  const uint64_t mb = (uint64_t)m->h.leaf_words * 8, cb = (uint64_t)m->h.leaf_size * m->h.value_bytes;
+  void *dorig, *dm, *dc, *de;
+  BXG_TRY(stage_in(c, 0, origins, n * 12, &dorig));
+  BXG_TRY(stage_in(c, 1, nullptr, n * mb, &dm));
+  BXG_TRY(stage_in(c, 2, nullptr, std::max<uint64_t>(n * cb, 64), &dc));
+  BXG_TRY(stage_in(c, 3, nullptr, n, &de));
+  BXG_TRY(launch(c, "k_read_leaves", k_read_leaves, (uint32_t)n, kBlock, (const MapDev*)m->d,
+                 (const int32_t*)dorig, (uint32_t)n, (uint64_t*)dm, (uint8_t*)dc, (uint8_t*)de));
+  BXG_HIP(c, hipMemcpyAsync(masks, dm, n * mb, hipMemcpyDeviceToHost, c->stream));
+  if (cb) BXG_HIP(c, hipMemcpyAsync(cells, dc, n * cb, hipMemcpyDeviceToHost, c->stream));
+  BXG_HIP(c, hipMemcpyAsync(exists, de, n, hipMemcpyDeviceToHost, c->stream));
+  return ctx_sync(c);
+}
+
+int bxg_write_leaves_host(bxg_map* m, const int32_t* origins, uint64_t n, const uint64_t* masks,
+                          const void* cells) {
+  if (!m) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (!n) return BXG_OK;
+  if (!origins || !masks || (m->h.value_bytes && !cells) || n >= (1u << 31))
+    return fail(c, BXG_ERR_INVALID_ARG, "bad leaf write arguments");
+  BXG_TRY(map_failed(m));
+  const uint64_t mb = (uint64_t)m->h.leaf_words * 8, cb = (uint64_t)m->h.leaf_size * m->h.value_bytes;
+  void *dorig, *dm, *dc = nullptr;
+  BXG_TRY(stage_in(c, 0, origins, n * 12, &dorig));
+  BXG_TRY(stage_in(c, 1, masks, n * mb, &dm));
+  if (cb) BXG_TRY(stage_in(c, 2, cells, n * cb, &dc));
+  for (int attempt = 0;; ++attempt) {
+    BXG_TRY(ensure_room(m, n, std::min<uint64_t>(n, 1u << 16)));
+    BXG_TRY(launch(c, "k_touch", k_touch, nblocks(n), kBlock, (const MapDev*)m->d,
+                   (const int32_t*)dorig, n));
+    BXG_TRY(read_counters(m));
+    if (!m->h_ctr->status) break;
+    if (m->h_ctr->status & kStCoordRange) {
+      BXG_TRY(clear_status(m));
+      return fail(c, BXG_ERR_COORD_RANGE, "coordinate outside the supported range");
+    }
+    if (attempt > 8) return fail(c, BXG_ERR_OUT_OF_POOL, "could not grow the node pool");
+    BXG_TRY(grow_after_overflow(m));
+  }
+  BXG_TRY(launch(c, "k_write_leaves", k_write_leaves, (uint32_t)n, kBlock, (const MapDev*)m->d,
+                 (const int32_t*)dorig, (uint32_t)n, (const uint64_t*)dm, (const uint8_t*)dc));
+  BXG_TRY(snapshot_counters(m));
+  return ctx_sync(c);
+}
+
+int bxg_leaf_origins_host(bxg_map* m, int32_t* origins, uint64_t cap, uint64_t* count) {
+  if (!m || !count) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  uint32_t nl = 0;
+  BXG_TRY(known_leaf_count(m, &nl));
+  *count = nl;
+  if (!origins || !cap || !nl) return nl > cap ? fail(c, BXG_ERR_CAPACITY, "origin buffer too small") : BXG_OK;
+  void* d;
+  BXG_TRY(dstage(c, 0, (uint64_t)nl * 12, &d));
+  BXG_TRY(launch(c, "k_leaf_origins", k_leaf_origins, nblocks(nl), kBlock, (const MapDev*)m->d, nl,
+                 (int32_t*)d));
+  BXG_TRY(d2h_sync(c, origins, d, std::min<uint64_t>(nl, cap) * 12));
+  return nl > cap ? fail(c, BXG_ERR_CAPACITY, "origin buffer too small") : BXG_OK;
+}
+
+// ------------------------------------------------------ probabilistic
+
+int bxg_set_options(bxg_map* m, const bxg_prob_options* o) {
+  if (!m || !o) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  m->h.opt_miss = o->prob_miss_log;
+  m->h.opt_hit = o->prob_hit_log;
+  m->h.opt_cmin = o->clamp_min_log;
+  m->h.opt_cmax = o->clamp_max_log;
+  m->h.opt_thr = o->occupancy_threshold_log;
+  return push_header(m);
+}
+int bxg_get_options(bxg_map* m, bxg_prob_options* o) {
+  if (!m || !o) return BXG_ERR_INVALID_ARG;
+  o->prob_miss_log = m->h.opt_miss;
+  o->prob_hit_log = m->h.opt_hit;
+  o->clamp_min_log = m->h.opt_cmin;
+  o->clamp_max_log = m->h.opt_cmax;
+  o->occupancy_threshold_log = m->h.opt_thr;
+  return BXG_OK;
+}
+int bxg_get_update_count(bxg_map* m, uint32_t* out) {
+  if (!m || !out) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(read_counters(m));
+  *out = (uint32_t)m->h_ctr->cur;
+  return BXG_OK;
+}
+int bxg_set_update_count(bxg_map* m, uint32_t v) {
+  if (!m || v < 1 || v > 3) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  const int32_t cur = (int32_t)v;
+  BXG_HIP(m->ctx, hipMemcpyAsync(&m->ctr->cur, &cur, 4, hipMemcpyHostToDevice, m->ctx->stream));
+  return ctx_sync(m->ctx);
+}
+
+int bxg_insert_point_cloud(bxg_map* m, const void* points, uint64_t n, uint32_t flags,
+                           const double origin[3], double max_range) {
+  if (!m || !origin) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  bxg_scan_desc s;
+  std::memset(&s, 0, sizeof(s));
+  s.map = m;
+  s.points_dev = points;
+  s.n = n;
+  s.flags = flags;
+  std::memcpy(s.origin, origin, sizeof(s.origin));
+  s.max_range = max_range;
+  BXG_TRY(run_batch(c, &s, 1, 0));
+  // rays due from addHit/MissPoint are traced by this insert
+  if (!m->extra.empty()) {
+    BXG_TRY(map_status(m));
+    m->extra.clear();
+  }
+  return BXG_OK;
+}
+
+// Run one wave, then re-run (alone, after growing its pools) every scan
+// whose map ran out of nodes; a failed scan left its map's cells untouched.
+static int run_wave_checked(bxg_ctx* c, const bxg_scan_desc* scans, uint32_t nscans) {
+  BXG_TRY(run_wave(c, scans, nscans, 0));
+  BXG_TRY(ctx_sync(c));
+  for (uint32_t s = 0; s < nscans; ++s) {
+    bxg_map* m = scans[s].map;
+    for (int attempt = 0;; ++attempt) {
+      BXG_TRY(read_counters(m));
+      const uint32_t st = m->h_ctr->status;
+      if (!st) break;
+      if (st & (kStCoordRange | kStScratchFull)) {
+        BXG_TRY(clear_status(m));
+        return fail(c, status_code(st), "insert failed: coordinate outside the supported range");
+      }
+      if (attempt > 8) return fail(c, BXG_ERR_OUT_OF_POOL, "could not grow the node pool");
+      BXG_TRY(grow_after_overflow(m));
+      BXG_TRY(run_wave(c, scans + s, 1, 0));
+    }
+    m->extra.clear();
+  }
+  return BXG_OK;
+}
+
+int bxg_insert_batch(bxg_ctx* c, const bxg_scan_desc* scans, uint32_t nscans) {
+  if (!c || (nscans && !scans)) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  for (uint32_t s = 0; s < nscans; ++s) {
+    if (!scans[s].map) return fail(c, BXG_ERR_INVALID_ARG, "scan without a map");
+    BXG_TRY(map_failed(scans[s].map));
+  }
+  uint32_t start = 0;
+  while (start < nscans) {
+    uint32_t end = start;
+    std::vector<bxg_map*> seen;
+    while (end < nscans && std::find(seen.begin(), seen.end(), scans[end].map) == seen.end()) {
+      seen.push_back(scans[end].map);
+      ++end;
+    }
+    BXG_TRY(run_wave_checked(c, scans + start, end - start));
+    start = end;
+  }
+  return BXG_OK;
+}
+
+int bxg_insert_point_cloud_host(bxg_map* m, const void* points, uint64_t n, uint32_t flags,
+                                const double origin[3], double max_range) {
+  if (!m || !origin || (n && !points)) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(map_failed(m));
+  const uint64_t bytes = n * ((flags & BXG_POINTS_F64) ? 24 : 12);
+  void* dp;
+  BXG_TRY(stage_in(c, 3, points, bytes, &dp));
+  bxg_scan_desc s;
+  std::memset(&s, 0, sizeof(s));
+  s.map = m;
+  s.points_dev = dp;
+  s.n = n;
+  s.flags = flags;
+  std::memcpy(s.origin, origin, sizeof(s.origin));
+  s.max_range = max_range;
+  for (int attempt = 0;; ++attempt) {
+    BXG_TRY(run_batch(c, &s, 1, 0));
+    BXG_TRY(read_counters(m));
+    const uint32_t st = m->h_ctr->status;
+    if (!st) break;
+    if (st & (kStCoordRange | kStScratchFull)) {
+      BXG_TRY(clear_status(m));
+      return fail(c, status_code(st), "insertPointCloud failed: coordinate outside the supported range");
+    }
+    if (attempt > 8) return fail(c, BXG_ERR_OUT_OF_POOL, "could not grow the node pool");
+    // nothing was written to any cell; grow and run the scan again
+    BXG_TRY(grow_after_overflow(m));
+  }
+  m->extra.clear();
+  return BXG_OK;
+}
+
+int bxg_add_point_host(bxg_map* m, const double xyz[3], int is_hit) {
+  if (!m || !xyz) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (!is_prob(m)) return fail(c, BXG_ERR_INVALID_ARG, "addHit/MissPoint needs a probabilistic map");
+  BXG_TRY(map_failed(m));
+  int32_t e[3];
+  BXG_TRY(origin_voxel(m, xyz, e));  // posToCoord with the range rule
+  void* dp;
+  BXG_TRY(stage_in(c, 3, xyz, 24, &dp));
+  bxg_scan_desc s;
+  std::memset(&s, 0, sizeof(s));
+  s.map = m;
+  s.points_dev = dp;
+  s.n = 1;
+  s.flags = BXG_POINTS_F64;
+  std::memcpy(s.origin, xyz, sizeof(s.origin));
+  s.max_range = 0.0;
+  for (int attempt = 0;; ++attempt) {
+    BXG_TRY(read_counters(m));
+    const uint64_t before = m->h_ctr->stat_endpoint_updates;
+    BXG_TRY(run_batch(c, &s, 1, kScanNoRays | (is_hit ? kScanForceHit : kScanForceMiss)));
+    BXG_TRY(read_counters(m));
+    const uint32_t st = m->h_ctr->status;
+    if (!st) {
+      if (m->h_ctr->stat_endpoint_updates != before) m->extra.push_back(make_int4(e[0], e[1], e[2], 0));
+      break;
+    }
+    if (st & (kStCoordRange | kStScratchFull)) {
+      BXG_TRY(clear_status(m));
+      return fail(c, status_code(st), "addHit/MissPoint failed");
+    }
+    if (attempt > 8) return fail(c, BXG_ERR_OUT_OF_POOL, "could not grow the node pool");
+    BXG_TRY(grow_after_overflow(m));
+  }
+  return BXG_OK;
+}
+
+// ---------------------------------------------------------- graphs
+
+int bxg_graph_create(bxg_ctx* c, uint32_t max_scans, uint64_t max_points, bxg_graph** out) {
+  if (!c || !out || !max_scans || !max_points) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  *out = nullptr;
+  if (max_points >= (1ull << 31) || max_scans >= 65535)
+    return fail(c, BXG_ERR_INVALID_ARG, "graph capacity too large");
+  bxg_graph* g = new bxg_graph();
+  g->ctx = c;
+  g->max_scans = max_scans;
+  g->max_points = max_points;
+  const uint64_t cap = pow2_at_least(std::max<uint64_t>(max_points, 4096));
+  const uint64_t tcap = cap * 2;
+  void* p = nullptr;
+  int rc = dmalloc(c, &p, cap * sizeof(int4));
+  g->sc.rec = (int4*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, cap * 4), g->sc.tpos = (uint32_t*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, cap * 4), g->sc.winners = (uint32_t*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, tcap * 8), g->sc.ep_keys = (uint64_t*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, tcap * 4), g->sc.ep_first = (uint32_t*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, 64), g->d_counters = (uint32_t*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, (uint64_t)max_scans * sizeof(ScanDesc)), g->d_desc = (ScanDesc*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, sizeof(BatchHdr)), g->d_hdr = (BatchHdr*)p;
+  g->sc.ep_mask = (uint32_t)(tcap - 1);
+  g->sc.counters = g->d_counters;
+  if (rc == BXG_OK) rc = fill64(c, g->sc.ep_keys, tcap, kEmptyKey);
+  if (rc == BXG_OK) rc = fill32(c, g->sc.ep_first, tcap, kNone);
+  if (rc == BXG_OK) rc = fill32(c, g->d_counters, 16, 0);
+  if (rc == BXG_OK) {
+    BatchHdr hdr{0, 0, 0};
+    rc = h2d_sync(c, g->d_hdr, &hdr, sizeof(hdr));
+  }
+  if (rc == BXG_OK) {
+    // capture the pipeline once; the kernels read the frame size from d_hdr
+    const bool prof = c->profiling;
+    c->profiling = false;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      rc = hip_fail(c, e, "hipStreamBeginCapture");
+    } else {
+      rc = enqueue_scan_kernels(c, g->d_hdr, g->d_desc, g->sc, max_points, max_scans, false);
+      hipGraph_t graph = nullptr;
+      e = hipStreamEndCapture(c->stream, &graph);
+      if (rc == BXG_OK && e != hipSuccess) rc = hip_fail(c, e, "hipStreamEndCapture");
+      g->graph = graph;
+      if (rc == BXG_OK) {
+        e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = hip_fail(c, e, "hipGraphInstantiate");
+      }
+    }
+    c->profiling = prof;
+  }
+  if (rc != BXG_OK) {
+    bxg_graph_destroy(g);
+    return rc;
+  }
+  *out = g;
+  return BXG_OK;
+}
+
+int bxg_graph_set_scans(bxg_graph* g, const bxg_scan_desc* scans, uint32_t nscans) {
+  if (!g || (nscans && !scans)) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = g->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (nscans > g->max_scans) return fail(c, BXG_ERR_INVALID_ARG, "more scans than the graph holds");
+  std::vector<ScanDesc> descs;
+  uint64_t total = 0;
+  BXG_TRY(build_descs(c, scans, nscans, 0u, descs, &total));
+  if (total > g->max_points) return fail(c, BXG_ERR_INVALID_ARG, "more points than the graph holds");
+  std::vector<bxg_map*> maps;
+  for (uint32_t s = 0; s < nscans; ++s) {
+    bxg_map* m = scans[s].map;
+    if (std::find(maps.begin(), maps.end(), m) != maps.end())
+      return fail(c, BXG_ERR_INVALID_ARG, "a graph frame holds one scan per map");
+    if (!m->extra.empty())
+      return fail(c, BXG_ERR_NOT_SUPPORTED, "graph frames do not trace addHit/MissPoint rays");
+    descs[s].extra = nullptr;
+    descs[s].nextra = 0;
+    maps.push_back(m);
+  }
+  BatchHdr hdr{nscans, 0, total};
+  BXG_TRY(upload(c, g->d_desc, descs.data(), descs.size() * sizeof(ScanDesc)));
+  BXG_TRY(upload(c, g->d_hdr, &hdr, sizeof(hdr)));
+  g->maps = maps;
+  g->loaded = true;
+  return BXG_OK;
+}
+
+int bxg_graph_launch(bxg_graph* g) {
+  if (!g) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = g->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (!g->loaded) return fail(c, BXG_ERR_INVALID_ARG, "no frame loaded");
+  for (bxg_map* m : g->maps) BXG_TRY(ensure_room(m, 1u << 14, 1024));
+  BXG_HIP(c, hipGraphLaunch(g->exec, c->stream));
+  for (bxg_map* m : g->maps) BXG_TRY(snapshot_counters(m));
+  return BXG_OK;
+}
+
+int bxg_graph_destroy(bxg_graph* g) {
+  if (!g) return BXG_ERR_INVALID_ARG;
+  bxg_ctx* c = g->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  hipStreamSynchronize(c->stream);
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  dfree(g->sc.rec);
+  dfree(g->sc.tpos);
+  dfree(g->sc.winners);
+  dfree(g->sc.ep_keys);
+  dfree(g->sc.ep_first);
+  dfree(g->d_counters);
+  dfree(g->d_desc);
+  dfree(g->d_hdr);
+  delete g;
+  return BXG_OK;
+}
+
+// ------------------------------------------------------------ checks
+
+int bxg_fp64_probe(bxg_ctx* c, const double* a, const double* b, const double* cc, uint64_t n,
+                   double* out) {
+  if (!c || (n && (!a || !b || !cc || !out))) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (!n) return BXG_OK;
+  BXG_TRY(launch(c, "k_fp64_probe", k_fp64_probe, nblocks(n), kBlock, a, b, cc, n, out));
+  return ctx_sync(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,178 @@
+// bxg_internal.h -- device-side layout of a sparse voxel map on one MI355X
+// and the structures shared by the kernels and the host runtime.
+//
+// One map (the counterpart of Bonxai::VoxelGrid<T>, bonxai.hpp:114-333):
+//
+//   root directory   open-addressing table of root keys, root_cap slots.
+//                    A root key is (coord >> log2n) per axis, 21 bits each
+//                    (biased by 2^20), packed into one 64-bit word so a slot
+//                    is claimed with a single 64-bit atomicCAS.  The slot
+//                    index IS the inner node: children[slot * inner_size + i]
+//                    holds the leaf index of inner cell i, or EMPTY.
+//   leaf pool        leaf_cap leaves handed out by an atomic bump counter:
+//                    leaf_masks[leaf * leaf_words + w]  (bit n of a leaf is
+//                    word n >> 6, bit n & 63, as mask.hpp:329-365)
+//                    cells[(leaf * leaf_size + n) * value_bytes]
+//                    leaf_origin[leaf] = min-corner coordinate of the leaf.
+//
+// Invariant kept by every kernel: a cell whose mask bit is off holds all-zero
+// bytes.  The reference never reads an off cell's storage (value() returns
+// null, setValue/setCellOn/value(create) overwrite it, forEachCell and
+// Serialize skip it), so the invariant is unobservable, and it lets the
+// scan-integration kernels derive "cell was off" from "cell word is zero".
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bxg {
+
+constexpr uint64_t kEmptyKey = ~0ull;
+constexpr uint32_t kEmptyChild = 0xFFFFFFFFu;
+constexpr uint32_t kPendingChild = 0xFFFFFFFEu;
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr int32_t kRootBias = 1 << 20;  // root key field bias (21-bit fields)
+
+// device status bits (sticky until the host clears them)
+enum : uint32_t {
+  kStOutOfPool = 1u,
+  kStRootFull = 2u,
+  kStCoordRange = 4u,
+  kStScratchFull = 8u,
+};
+
+// map flags
+enum : uint32_t {
+  kMapProbabilistic = 1u,
+};
+
+struct alignas(64) MapCounters {
+  uint32_t leaf_count;  // bump allocator (may run past leaf_cap on failure)
+  uint32_t root_count;
+  uint32_t status;
+  int32_t cur;  // ProbabilisticMap update_count, 1..3
+  uint64_t scans_applied;
+  uint64_t stat_endpoint_updates;
+  uint64_t stat_rays;
+  uint64_t stat_ray_updates;
+  uint64_t pad[2];
+};
+
+struct alignas(64) MapDev {
+  double inv_res;
+  double res;
+  uint32_t inner_bits, leaf_bits, log2n, inner_size;
+  uint32_t leaf_size, leaf_words, value_bytes, flags;
+  uint64_t* root_keys;
+  uint32_t* children;
+  uint32_t root_mask;  // root_cap - 1
+  uint32_t max_probe;
+  uint64_t* leaf_masks;
+  uint8_t* cells;
+  int4* leaf_origin;
+  uint32_t leaf_cap;
+  int32_t opt_miss, opt_hit, opt_cmin, opt_cmax, opt_thr;
+  MapCounters* ctr;
+};
+
+// one scan of a (possibly batched) insertPointCloud call
+struct alignas(16) ScanDesc {
+  const MapDev* map;
+  const void* points;  // xyz triples, float32 or float64 (see flags)
+  uint64_t pt_off;     // first flattened point index of this scan
+  uint32_t n;
+  uint32_t flags;  // kScan* bits
+  double fx, fy, fz;
+  double max_range, r2;
+  int32_t ox, oy, oz;  // origin voxel, posToCoord(origin)
+  uint32_t tag;        // scan index inside the batch (endpoint table key)
+  const int4* extra;   // endpoints added by addHit/MissPoint whose rays are due
+  uint32_t nextra;
+  uint32_t pad;
+};
+
+enum : uint32_t {
+  kScanF64 = 1u,        // points are float64
+  kScanForceHit = 2u,   // addHitPoint: no max-range clip, kind hit
+  kScanForceMiss = 4u,  // addMissPoint: no max-range clip, kind miss
+  kScanNoRays = 8u,     // endpoint update only, keep update_count
+};
+
+// size of a batch, read by the kernels from device memory so that a captured
+// graph can replay any batch up to the capacity it was built for
+struct alignas(16) BatchHdr {
+  uint32_t nscans;
+  uint32_t pad;
+  uint64_t total;
+};
+
+// scratch of one integration call, owned by the context
+struct ScanScratch {
+  int4* rec;            // per flattened point: endpoint xyz, flags | scan << 16
+  uint32_t* tpos;       // per point: endpoint-table slot of its endpoint
+  uint64_t* ep_keys;    // endpoint table keys
+  uint32_t* ep_first;   // endpoint table: min point index
+  uint32_t ep_mask;     // capacity - 1
+  uint32_t* winners;    // flattened point indices of per-voxel winners
+  uint32_t* counters;   // [0] winner count, [1] ray work cursor
+};
+
+// rec.w flag bits
+enum : uint32_t {
+  kRecHit = 1u,
+  kRecMarked = 2u,
+  kRecValid = 4u,
+  kRecWinner = 8u,
+};
+
+// ------------------------------------------------------------ helpers
+
+__host__ __device__ inline uint64_t mix64(uint64_t h) {
+  h ^= h >> 33;
+  h *= 0xFF51AFD7ED558CCDull;
+  h ^= h >> 33;
+  h *= 0xC4CEB9FE1A85EC53ull;
+  h ^= h >> 33;
+  return h;
+}
+
+// root key of coordinate (x, y, z); false when a field leaves 21 bits
+__host__ __device__ inline bool pack_root(int32_t x, int32_t y, int32_t z, uint32_t log2n,
+                                          uint64_t* key) {
+  const int32_t rx = x >> log2n, ry = y >> log2n, rz = z >> log2n;
+  const uint32_t ux = (uint32_t)(rx + kRootBias), uy = (uint32_t)(ry + kRootBias),
+                 uz = (uint32_t)(rz + kRootBias);
+  if ((ux | uy | uz) >= (1u << 21)) return false;
+  *key = (uint64_t)ux | ((uint64_t)uy << 21) | ((uint64_t)uz << 42);
+  return true;
+}
+
+__host__ __device__ inline void unpack_root(uint64_t key, uint32_t log2n, int32_t* x, int32_t* y,
+                                            int32_t* z) {
+  *x = ((int32_t)(key & 0x1FFFFF) - kRootBias) * (1 << log2n);
+  *y = ((int32_t)((key >> 21) & 0x1FFFFF) - kRootBias) * (1 << log2n);
+  *z = ((int32_t)((key >> 42) & 0x1FFFFF) - kRootBias) * (1 << log2n);
+}
+
+// getInnerIndex, bonxai.hpp:431-438
+__host__ __device__ inline uint32_t inner_index(int32_t x, int32_t y, int32_t z, uint32_t lb,
+                                                uint32_t ib) {
+  const uint32_t im = (1u << ib) - 1;
+  return ((uint32_t)(x >> lb) & im) | (((uint32_t)(y >> lb) & im) << ib) |
+         (((uint32_t)(z >> lb) & im) << (2 * ib));
+}
+
+// getLeafIndex, bonxai.hpp:440-447
+__host__ __device__ inline uint32_t leaf_index(int32_t x, int32_t y, int32_t z, uint32_t lb) {
+  const uint32_t lm = (1u << lb) - 1;
+  return ((uint32_t)x & lm) | (((uint32_t)y & lm) << lb) | (((uint32_t)z & lm) << (2 * lb));
+}
+
+// the cell hash of the order-free digest (same definition as the oracle's)
+__host__ __device__ inline uint64_t cell_hash(int32_t x, int32_t y, int32_t z, uint64_t word) {
+  const uint64_t a = (uint64_t)(uint32_t)x | ((uint64_t)(uint32_t)y << 32);
+  const uint64_t b = (uint64_t)(uint32_t)z ^ (word * 0x9E3779B97F4A7C15ull);
+  return mix64(a ^ mix64(b + 0x632BE59BD9B4E019ull));
+}
+
+}  // namespace bxg

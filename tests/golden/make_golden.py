@@ -1,0 +1,221 @@
+"""Writes the golden fixtures of tests/golden/ (run in the development
+container, where the reference tree and oracle/_ref exist):
+
+  apple_xyz.npy   x, y, z of the reference's data/apple.pcd (float32, parsed
+                  from its ASCII rows) -- the only real-data input the
+                  reference holds
+  kat.npz         cell dumps (sorted coords, raw CellT words, update_count)
+                  of small scan-integration scenarios, computed by scan
+                  integration over the reference's own VoxelGrid<CellT>
+                  (oracle/_ref) and checked against the restatement
+  grid.npz        VoxelGrid<float> results of the reference's own VoxelGrid
+                  on a reduced clustered workload (was_on, lookups, cells)
+  digests.json    input digests of every synthetic workload and order-free
+                  map digests of the full-size workloads (restatement,
+                  cross-checked against the reference's VoxelGrid where that
+                  is feasible)
+
+Usage: python tests/golden/make_golden.py [--full]
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from bonxai_amd import scenes  # noqa: E402
+
+REF_DATA = "/root/reference/data/apple.pcd"
+
+
+def read_pcd_xyz(path):
+    with open(path) as f:
+        lines = f.read().splitlines()
+    start = next(i for i, l in enumerate(lines) if l.startswith("DATA")) + 1
+    rows = [l.split() for l in lines[start:] if l.strip()]
+    return np.array([[float(r[0]), float(r[1]), float(r[2])] for r in rows], np.float64).astype(np.float32)
+
+
+def dump_prob_ref(scans, res):
+    """Scan integration over the reference VoxelGrid (and the restatement,
+    which must agree) for a list of (points, origin, max_range)."""
+    rp = oracle.RefProb(res)
+    op = oracle.ProbMap(res)
+    for pts, origin, R in scans:
+        rp.insert_point_cloud(pts, origin, R)
+        op.insert_point_cloud(pts, origin, R)
+    rc, rw = rp.cells()
+    oc, ow = op.cells()
+    assert np.array_equal(rc, oc) and np.array_equal(rw, ow), "restatement != reference-backed run"
+    assert rp.update_count == op.update_count
+    out = (rc, rw, rp.update_count)
+    rp.close()
+    op.close()
+    return out
+
+
+def kat_scenarios():
+    z = np.zeros(3)
+    out = {}
+    # apple.pcd at the origin, 5 mm voxels, max range 0.75 m (part clips)
+    apple = np.load(os.path.join(HERE, "apple_xyz.npy"))
+    out["apple"] = ([(apple, z, 0.75)], 0.005)
+    # a hit and a clipped miss in the same voxel, both orders
+    a = np.array([[1.04, 0.0, 0.0]], np.float32)
+    b = np.array([[5.0, 0.0, 0.0]], np.float32)
+    out["first_hit"] = ([(np.concatenate([a, b]), z, 1.05)], 0.1)
+    out["first_miss"] = ([(np.concatenate([b, a]), z, 1.05)], 0.1)
+    # a cell touched in scan k, untouched in k+1 and k+2, endpoint in k+3:
+    # its id equals the scan id again, so it is frozen and traces no ray
+    p0 = np.array([[0.55, 0.0, 0.0]], np.float32)
+    far = np.array([[0.0, 3.05, 0.0]], np.float32)
+    out["frozen"] = ([(p0, z, 5.0), (far, z, 5.0), (far, z, 5.0),
+                      (np.concatenate([p0, far]), z, 5.0)], 0.1)
+    # seven sequential scans from moving origins (full id cycle)
+    seq = []
+    for k in range(7):
+        pts, origin = scenes.lidar_scan(3, 50 + k, float(k) * 1.5, 16, 64, cast_range=10.0,
+                                        far_range=9.0)
+        seq.append((pts, origin, 6.0))
+    out["cycle7"] = (seq, 0.1)
+    # an empty scan between two scans
+    pts, origin = scenes.box_scan(seed=3, n=2000, size=(3.0, 2.0, 1.5))
+    out["empty"] = ([(pts, origin, 10.0), (pts[:0], origin, 10.0), (pts, origin, 10.0)], 0.1)
+    # a point in the origin voxel (endpoint without a ray)
+    out["origin_voxel"] = ([(np.array([[0.01, 0.02, 0.03], [0.5, 0.5, 0.5]], np.float32), z, 5.0)], 0.1)
+    # negative coordinates and root crossings in every octant
+    rng = np.random.default_rng(3)
+    pts = ((rng.random((1500, 3)) - 0.5) * 8.0).astype(np.float32)
+    out["octants"] = ([(pts, np.array([0.013, -0.021, 0.004]), 3.3)], 0.1)
+    return out
+
+
+def make_kat():
+    arrays = {}
+    for name, (scans, res) in kat_scenarios().items():
+        c, w, cur = dump_prob_ref(scans, res)
+        arrays[f"{name}_coords"] = c
+        arrays[f"{name}_words"] = w
+        arrays[f"{name}_cur"] = np.array([cur], np.int32)
+        print(f"kat {name}: {len(c)} cells, update_count {cur}")
+    np.savez_compressed(os.path.join(HERE, "kat.npz"), **arrays)
+
+
+def make_grid():
+    coords = scenes.clusters(n=100_000)
+    vals = scenes.cluster_values(coords)
+    r = oracle.RefGrid(0.05, np.float32)
+    o = oracle.Grid(0.05, dtype=np.float32)
+    was = r.set_values(coords, vals)
+    assert np.array_equal(was, o.set_values(coords, vals))
+    q = scenes.lookups(coords, n=200_000)
+    v, f = r.lookup(q)
+    ov, of = o.lookup(q)
+    assert np.array_equal(v, ov) and np.array_equal(f, of)
+    c, cv = r.export()
+    oc, ocv = o.export()
+    assert np.array_equal(c, oc) and np.array_equal(cv, ocv)
+    np.savez_compressed(os.path.join(HERE, "grid.npz"), was_on=was, lookup_vals=v, lookup_found=f,
+                        cells=c, cell_vals=cv)
+    print(f"grid: {len(c)} cells")
+
+
+def fnv(a):
+    return scenes.fnv1a(np.ascontiguousarray(a))
+
+
+def make_digests(full):
+    d = {"inputs": {}, "maps": {}}
+    t0 = time.time()
+    pts, origin = scenes.box_scan()
+    d["inputs"]["box_scan"] = fnv(pts)
+    m = oracle.ProbMap(0.05)
+    m.insert_point_cloud(pts, origin, 10.0)
+    rp = oracle.RefProb(0.05)
+    rp.insert_point_cloud(pts, origin, 10.0)
+    c, w = rp.cells()
+    oc, ow = m.cells()
+    assert np.array_equal(c, oc) and np.array_equal(w, ow)
+    d["maps"]["box_scan"] = {"digest": list(map(str, m.digest())), "stats": m.stats(),
+                             "update_count": m.update_count, "leaves": m.grid.leaf_count(),
+                             "roots": m.grid.root_count()}
+    m.close()
+    rp.close()
+    print("box", time.time() - t0)
+
+    coords = scenes.clusters()
+    vals = scenes.cluster_values(coords)
+    q = scenes.lookups(coords)
+    d["inputs"]["clusters"] = fnv(coords)
+    d["inputs"]["lookups"] = fnv(q)
+    g = oracle.Grid(0.05, dtype=np.float32)
+    was = g.set_values(coords, vals)
+    v, f = g.lookup(q)
+    if full:
+        r = oracle.RefGrid(0.05, np.float32)
+        assert np.array_equal(was, r.set_values(coords, vals))
+        rv, rf = r.lookup(q)
+        assert np.array_equal(rv, v) and np.array_equal(rf, f)
+        r.close()
+    d["maps"]["clusters"] = {"digest": list(map(str, g.digest())), "was_on": fnv(was.astype(np.uint8)),
+                             "lookup_vals": fnv(v), "lookup_found": fnv(f.astype(np.uint8)),
+                             "distinct": int(g.active_cells_count()), "leaves": g.leaf_count(),
+                             "roots": g.root_count(), "hits": int(f.sum())}
+    g.close()
+    print("clusters", time.time() - t0)
+
+    nseq = 200 if full else 10
+    m = oracle.ProbMap(0.1)
+    seq_d = []
+    for k, (pts, origin) in enumerate(scenes.lidar_sequence(nseq)):
+        if k < 3:
+            d["inputs"][f"lidar_{k}"] = fnv(pts)
+        m.insert_point_cloud(pts, origin, 50.0)
+        st = m.stats()
+        seq_d.append({"scan": k, "digest": list(map(str, m.digest())), "stats": st})
+        if k % 10 == 0:
+            print("lidar", k, time.time() - t0, flush=True)
+    d["maps"]["lidar_sequence"] = seq_d
+    m.close()
+
+    sizes = scenes.batch_sizes()
+    d["inputs"]["batch_sizes"] = [int(x) for x in sizes]
+    batch = scenes.batch_scans()
+    maps = []
+    for i, (pts, origin) in enumerate(batch):
+        d["inputs"][f"batch_{i}"] = fnv(pts)
+        if full or i < 8:
+            m = oracle.ProbMap(0.1)
+            m.insert_point_cloud(pts, origin, 30.0)
+            maps.append({"map": i, "digest": list(map(str, m.digest())), "stats": m.stats()})
+            m.close()
+    d["maps"]["batch"] = maps
+    print("batch", time.time() - t0)
+
+    for s in range(4):
+        pts, origin = scenes.depth_frame(s)
+        d["inputs"][f"depth_{s}"] = fnv(pts)
+    return d
+
+
+def main():
+    full = "--full" in sys.argv
+    if not os.path.exists(os.path.join(HERE, "apple_xyz.npy")):
+        np.save(os.path.join(HERE, "apple_xyz.npy"), read_pcd_xyz(REF_DATA))
+    make_kat()
+    make_grid()
+    d = make_digests(full)
+    d["full"] = full
+    with open(os.path.join(HERE, "digests.json"), "w") as f:
+        json.dump(d, f, indent=1, sort_keys=True)
+    print("done")
+
+
+if __name__ == "__main__":
+    main()

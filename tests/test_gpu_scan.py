@@ -1,0 +1,328 @@
+"""GPU parity of ProbabilisticMap scan integration against the oracle.
+
+Bit-exact bar: the same allocated (on) coordinate set, the same raw CellT
+word ((log-odds << 4) | update_id) in every cell, and the same update_count,
+after every scan.
+"""
+import numpy as np
+import pytest
+
+from conftest import assert_same_cells
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle():
+    import oracle
+    return oracle
+
+
+def _gpu_cells(m):
+    c, w = m.cells()
+    return c, w
+
+
+def test_fp64_front_end_is_ieee(ctx):
+    """sqrt, division, the unfused (a/b)*c + c, floor(a*b) and the ordered
+    squared norm of the scan front end round exactly as IEEE-754 double."""
+    rng = np.random.default_rng(5)
+    n = 1 << 20
+    a = np.concatenate([rng.random(n // 2) * 1e4, np.exp(rng.normal(0, 20, n // 2))])
+    b = np.concatenate([rng.random(n // 2) * 50 + 1e-3, np.exp(rng.normal(0, 5, n // 2))])
+    c = rng.normal(0, 30, n)
+    # values landing next to voxel boundaries: k*0.05 +- a few ulp
+    k = rng.integers(-10**6, 10**6, 4096).astype(np.float64) * 0.05
+    a[:4096] = np.nextafter(k, k + np.where(rng.random(4096) < 0.5, -1, 1))
+    b[:4096] = 20.0
+    out = ctx.fp64_probe(a, b, c)
+    with np.errstate(all="ignore"):
+        q = a / b
+        exp = np.stack([np.sqrt(a), q, c + (q * c), np.floor(a * b), (a * a + b * b) + c * c], 1)
+    for j, name in enumerate(["sqrt", "div", "c+(a/b)*c", "floor(a*b)", "sqnorm"]):
+        assert np.array_equal(out[:, j].view(np.uint64), exp[:, j].view(np.uint64)), name
+
+
+def test_box_scan_single(ctx):
+    """One 100k-point box-room scan at 0.05 m, max range 10 m."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    pts, origin = scenes.box_scan()
+    om = ora.ProbMap(0.05)
+    om.insert_point_cloud(pts, origin, 10.0)
+    m = bx.ProbabilisticMap(ctx, 0.05)
+    m.insert_point_cloud(pts, origin, 10.0)
+    assert_same_cells(_gpu_cells(m), om.cells(), "box scan")
+    assert m.update_count == om.update_count == 2
+    assert m.digest() == om.digest()
+    m.close()
+    om.close()
+
+
+def test_box_scan_clipped_and_repeated(ctx):
+    """Max range 2.5 m (most points clip to misses) and the same scan three
+    more times, through update_count 2 -> 3 -> 1 -> 2."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    pts, origin = scenes.box_scan(seed=9, n=30_000)
+    om = ora.ProbMap(0.05)
+    m = bx.ProbabilisticMap(ctx, 0.05)
+    for k in range(4):
+        om.insert_point_cloud(pts, origin, 2.5)
+        m.insert_point_cloud(pts, origin, 2.5)
+        assert_same_cells(_gpu_cells(m), om.cells(), f"scan {k}")
+        assert m.update_count == om.update_count
+    m.close()
+    om.close()
+
+
+def test_lidar_sequence_prefix(ctx):
+    """The first scans of the sequential LiDAR workload (0.1 m, 50 m), with a
+    full comparison after each scan."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    om = ora.ProbMap(0.1)
+    m = bx.ProbabilisticMap(ctx, 0.1)
+    for k, (pts, origin) in enumerate(scenes.lidar_sequence(3)):
+        om.insert_point_cloud(pts, origin, 50.0)
+        m.insert_point_cloud(pts, origin, 50.0)
+        assert m.digest() == om.digest(), f"scan {k}"
+    assert_same_cells(_gpu_cells(m), om.cells(), "after 3 scans")
+    m.close()
+    om.close()
+
+
+def test_id_cycle_and_frozen_cells(ctx):
+    """Eight small scans from alternating origins exercise the 3-value
+    update_id cycle, including cells frozen because their id equals the
+    scan's id (probabilistic_map.cpp:103-105)."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    om = ora.ProbMap(0.1)
+    m = bx.ProbabilisticMap(ctx, 0.1)
+    for k in range(8):
+        pts, origin = scenes.lidar_scan(3, 77 + k, float(k % 3) * 2.0, 16, 64, far_range=30.0)
+        rng_keep = np.random.default_rng(k).random(len(pts)) < (0.5 if k % 2 else 1.0)
+        pts = pts[rng_keep]
+        om.insert_point_cloud(pts, origin, 20.0)
+        m.insert_point_cloud(pts, origin, 20.0)
+        assert_same_cells(_gpu_cells(m), om.cells(), f"scan {k}")
+        assert m.update_count == om.update_count
+    m.close()
+    om.close()
+
+
+def test_kat_first_index_wins(ctx):
+    """A hit and a clipped miss landing in the same voxel: the first point in
+    input order decides the update (probabilistic_map.cpp:30-54)."""
+    import bonxai_amd as bx
+    origin = np.zeros(3)
+    R = 1.0
+    hit = np.array([0.99, 0.0, 0.0], np.float32)      # |v| < R: hit at x=0.99
+    miss = np.array([1.5, 0.0, 0.0], np.float32)      # clipped to x=1.0 -> voxel 10
+    near = np.array([0.999, 0.001, 0.0], np.float32)  # hit in voxel 9
+    for order, kind in (((hit, miss), "hit first"), ((miss, hit), "miss first")):
+        # voxel of the hit: floor(0.99*10)=9; of the clipped miss: floor(1.0*10)=10.
+        # Use a second hit in the miss voxel to share it.
+        pts = np.stack([order[0], order[1], near])
+        m = bx.ProbabilisticMap(ctx, 0.1)
+        m.insert_point_cloud(pts, origin, R)
+        om = _oracle().ProbMap(0.1)
+        om.insert_point_cloud(pts, origin, R)
+        assert_same_cells(_gpu_cells(m), om.cells(), kind)
+        m.close()
+        om.close()
+    # same voxel, both kinds: point at 1.04 (hit, R=1.05) and a far point
+    # clipped onto 1.05*(direction) -> both in voxel 10 at 0.1 m
+    R = 1.05
+    a = np.array([1.04, 0.0, 0.0], np.float32)
+    b = np.array([5.0, 0.0, 0.0], np.float32)
+    for pts, expect in ((np.stack([a, b]), 847297), (np.stack([b, a]), -405465)):
+        m = bx.ProbabilisticMap(ctx, 0.1)
+        m.insert_point_cloud(pts, origin, R)
+        c, w = m.cells()
+        sel = np.all(c == [10, 0, 0], axis=1)
+        assert sel.sum() == 1
+        assert (w[sel].astype(np.int32) >> 4)[0] == expect
+        m.close()
+
+
+def test_kat_empty_scan_and_origin_voxel(ctx):
+    """An empty scan still advances update_count; a point in the origin
+    voxel is an endpoint without a ray."""
+    import bonxai_amd as bx
+    m = bx.ProbabilisticMap(ctx, 0.1)
+    m.insert_point_cloud(np.zeros((0, 3), np.float32), np.zeros(3), 5.0)
+    assert m.update_count == 2
+    m.insert_point_cloud(np.array([[0.05, 0.05, 0.05]], np.float32), np.zeros(3), 5.0)
+    c, w = m.cells()
+    assert c.tolist() == [[0, 0, 0]]
+    assert int(w[0]) == ((847297 << 4) | 2)
+    assert m.update_count == 3
+    m.close()
+
+
+def test_negative_coords_and_root_crossings(ctx):
+    """Rays crossing root (32-voxel) and leaf boundaries in every octant."""
+    import bonxai_amd as bx
+    ora = _oracle()
+    rng = np.random.default_rng(3)
+    pts = (rng.random((20_000, 3)) - 0.5) * 14.0
+    origin = np.array([0.013, -0.021, 0.004])
+    om = ora.ProbMap(0.1)
+    m = bx.ProbabilisticMap(ctx, 0.1)
+    for k in range(2):
+        om.insert_point_cloud(pts.astype(np.float32), origin + k, 6.0)
+        m.insert_point_cloud(pts.astype(np.float32), origin + k, 6.0)
+    assert_same_cells(_gpu_cells(m), om.cells(), "octants")
+    m.close()
+    om.close()
+
+
+def test_double_points(ctx):
+    """insertPointCloud with double points (Eigen::Vector3d callers)."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    pts, origin = scenes.box_scan(seed=4, n=20_000)
+    p64 = pts.astype(np.float64) + 1e-9
+    om = ora.ProbMap(0.05)
+    om.insert_point_cloud(p64, origin, 4.0)
+    m = bx.ProbabilisticMap(ctx, 0.05)
+    m.insert_point_cloud(p64, origin, 4.0)
+    assert_same_cells(_gpu_cells(m), om.cells(), "f64 points")
+    m.close()
+    om.close()
+
+
+def test_pool_growth_replay(ctx):
+    """A scan that overflows a tiny node pool is re-run after the pool grows,
+    and the result is still exact (cells are untouched by the failed run)."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    pts, origin = scenes.box_scan(seed=5)
+    m = bx.ProbabilisticMap(ctx, 0.05)
+    st0 = m.stats()
+    assert st0["leaf_capacity"] < 5000
+    m.insert_point_cloud(pts, origin, 10.0)
+    om = ora.ProbMap(0.05)
+    om.insert_point_cloud(pts, origin, 10.0)
+    assert_same_cells(_gpu_cells(m), om.cells(), "after growth")
+    assert m.stats()["leaf_count"] == om.grid.leaf_count()
+    m.close()
+    om.close()
+
+
+def test_occupied_free_and_classify(ctx):
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    pts, origin = scenes.box_scan(seed=6, n=50_000)
+    om = ora.ProbMap(0.05)
+    om.insert_point_cloud(pts, origin, 10.0)
+    m = bx.ProbabilisticMap(ctx, 0.05)
+    m.insert_point_cloud(pts, origin, 10.0)
+    from conftest import sorted_cells
+    occ = sorted_cells(m.get_occupied_voxels())[0]
+    free = sorted_cells(m.get_free_voxels())[0]
+    assert np.array_equal(occ, om.occupied())
+    assert np.array_equal(free, om.free())
+    rng = np.random.default_rng(1)
+    q = np.concatenate([occ[:1000], free[:1000], rng.integers(-120, 120, (2000, 3))]).astype(np.int32)
+    assert np.array_equal(m.classify(q), om.classify(q))
+    m.close()
+    om.close()
+
+
+def test_add_hit_miss_points(ctx):
+    """addHitPoint / addMissPoint update the endpoint at once and trace the
+    ray on the next insertPointCloud (probabilistic_map.hpp:95-103)."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    pts, origin = scenes.box_scan(seed=8, n=5_000)
+    om = ora.ProbMap(0.05)
+    m = bx.ProbabilisticMap(ctx, 0.05)
+    for p, hit in (((1.0, 0.5, 0.2), True), ((-2.0, 1.0, 1.0), False), ((1.0, 0.5, 0.2), False)):
+        (om.add_hit_point if hit else om.add_miss_point)(np.array(p))
+        (m.add_hit_point if hit else m.add_miss_point)(p)
+    assert_same_cells(_gpu_cells(m), om.cells(), "after add points")
+    om.insert_point_cloud(pts, origin, 10.0)
+    m.insert_point_cloud(pts, origin, 10.0)
+    assert_same_cells(_gpu_cells(m), om.cells(), "after the next insert")
+    m.close()
+    om.close()
+
+
+def test_batch_matches_sequential(ctx):
+    """Eight uneven scans into eight maps in one batched pass equal eight
+    independent oracle runs."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    scans = scenes.batch_scans(nmaps=8, lo=2_000, hi=40_000)
+    maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+    dev = [ctx.to_device(p) for p, _ in scans]
+    bx.insert_batch(ctx, [(m, d, o, 30.0) for m, d, (_, o) in zip(maps, dev, scans)])
+    for i, (m, (p, o)) in enumerate(zip(maps, scans)):
+        assert m.sync() == 0
+        om = ora.ProbMap(0.1)
+        om.insert_point_cloud(p, o, 30.0)
+        assert_same_cells(_gpu_cells(m), om.cells(), f"map {i}")
+        om.close()
+    for m in maps:
+        m.close()
+    for d in dev:
+        d.free()
+
+
+def test_batch_same_map_keeps_order(ctx):
+    """Scans naming the same map in one batch run in input order."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    seq = list(scenes.lidar_sequence(3, nazim=128))
+    m = bx.ProbabilisticMap(ctx, 0.1)
+    other = bx.ProbabilisticMap(ctx, 0.1)
+    dev = [ctx.to_device(p) for p, _ in seq]
+    bx.insert_batch(ctx, [(m, dev[0], seq[0][1], 50.0), (other, dev[1], seq[1][1], 50.0),
+                          (m, dev[1], seq[1][1], 50.0), (m, dev[2], seq[2][1], 50.0)])
+    assert m.sync() == 0
+    om = ora.ProbMap(0.1)
+    for p, o in seq:
+        om.insert_point_cloud(p, o, 50.0)
+    assert_same_cells(_gpu_cells(m), om.cells(), "same-map batch")
+    assert m.update_count == om.update_count == 1
+    for x in (m, other):
+        x.close()
+    for d in dev:
+        d.free()
+
+
+def test_graph_replays_equal_eager(ctx):
+    """A captured frame graph replayed four times (with eager work and
+    synchronisation between replays) equals four eager batched inserts."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    scans = scenes.batch_scans(nmaps=6, lo=2_000, hi=20_000)
+    dev = [ctx.to_device(p) for p, _ in scans]
+    gmaps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+    emaps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+    g = bx.ScanGraph(ctx, 8, sum(len(p) for p, _ in scans))
+    g.set_scans([(m, d, o, 30.0) for m, d, (_, o) in zip(gmaps, dev, scans)])
+    for rep in range(4):
+        g.launch()
+        bx.insert_batch(ctx, [(m, d, o, 30.0) for m, d, (_, o) in zip(emaps, dev, scans)])
+        ctx.sync()
+        for i, (a, b) in enumerate(zip(gmaps, emaps)):
+            assert a.digest() == b.digest(), f"replay {rep} map {i}"
+            assert a.update_count == b.update_count
+    g.close()
+    for m in gmaps + emaps:
+        m.close()
+    for d in dev:
+        d.free()
