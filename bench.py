@@ -1,0 +1,293 @@
+#!/usr/bin/env python3
+"""Benchmark of scan integration (ProbabilisticMap::insertPointCloud) on one
+MI355X: the batched form (one captured graph per frame of 64 scans into 64
+maps) against the plain per-scan form (one insert call per scan), with the
+CPU restatement timed beside it.
+
+Workload (one "step" = one frame): 64 independent maps at 0.1 m voxels, each
+receiving one spinning-LiDAR scan of a seeded box scene, point counts
+log-uniform in 10k..500k (about 8M points a frame), max range 30 m.  Inputs
+are resident in HBM before the timed region; the same frame is integrated
+every step into maps that persist (steady state: every touched cell takes an
+update each frame, no cell is frozen).
+
+Prints one JSON line.  Extra fields: the plain per-scan rate, the
+batched/plain ratio, voxel updates/s, per-frame p50/p99 latency, the
+roofline of the ray kernel (k_scan_rays) and of the bulk lookup kernel
+(VoxelGrid<float>, 2M inserts / 8M lookups), and the CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "rays/s + voxel updates/s, batched vs plain per-scan on 1 MI355X; p50/p99 latency"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def frame_stats(maps):
+    rays = sum(m.stats()["last_rays"] for m in maps)
+    ray_upd = sum(m.stats()["last_ray_updates"] for m in maps)
+    return rays, ray_upd
+
+
+def run_batched(ctx, bx, scans, dev, steps, warmup, max_range):
+    maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+    descs = [(m, d, o, max_range) for m, d, (_, o) in zip(maps, dev, scans)]
+    # first frame through the blocking batched call: it sizes every pool
+    bx.insert_batch(ctx, descs)
+    for m in maps:
+        st = m.stats()
+        m.reserve(int(st["leaf_count"] * 1.25) + 8192, int(st["root_count"] * 2) + 1024)
+    # one counted frame: exact rays and cell updates per frame
+    ctx.counting(True)
+    bx.insert_batch(ctx, descs)
+    ctx.counting(False)
+    rays, ray_upd = frame_stats(maps)
+    total_points = sum(len(p) for p, _ in scans)
+    g = bx.ScanGraph(ctx, len(scans), total_points)
+    g.set_scans(descs)
+    for _ in range(warmup):
+        g.launch()
+    ctx.sync()
+    lat = []
+    ctx.event(0)
+    for _ in range(steps):
+        g.launch()
+    ctx.event(1)
+    total_ms = ctx.elapsed_ms(0, 1)
+    for m in maps:
+        if m.sync() != 0:
+            raise RuntimeError("a map failed during the timed frames")
+    # per-frame latency: frames timed one at a time
+    for k in range(max(steps, 10)):
+        ctx.event(2)
+        g.launch()
+        ctx.event(3)
+        lat.append(ctx.elapsed_ms(2, 3))
+    # kernel durations of the same frame, eager launches with per-kernel events
+    ctx.profile_reset()
+    ctx.profile(True)
+    nprof = 3
+    for _ in range(nprof):
+        bx.insert_batch(ctx, descs)
+    ctx.profile(False)
+    kern = {}
+    for name in ("k_scan_classify", "k_scan_winners", "k_scan_endpoints", "k_scan_rays",
+                 "k_scan_finalize"):
+        n, ms = ctx.profile_read(name)
+        kern[name] = ms / max(n, 1)
+    g.close()
+    for m in maps:
+        m.close()
+    return {
+        "total_ms": total_ms, "lat": lat, "rays_traced": rays, "ray_updates": ray_upd,
+        "kernels_ms": kern, "points": total_points,
+    }
+
+
+def run_plain(ctx, bx, scans, dev, steps, warmup, max_range):
+    maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+    bx.insert_batch(ctx, [(m, d, o, max_range) for m, d, (_, o) in zip(maps, dev, scans)])
+    for m in maps:
+        st = m.stats()
+        m.reserve(int(st["leaf_count"] * 1.25) + 8192, int(st["root_count"] * 2) + 1024)
+    for _ in range(warmup):
+        for m, d, (_, o) in zip(maps, dev, scans):
+            m.insert_point_cloud(d, o, max_range, sync=False)
+    ctx.sync()
+    ctx.event(0)
+    for _ in range(steps):
+        for m, d, (_, o) in zip(maps, dev, scans):
+            m.insert_point_cloud(d, o, max_range, sync=False)
+    ctx.event(1)
+    total_ms = ctx.elapsed_ms(0, 1)
+    for m in maps:
+        if m.sync() != 0:
+            raise RuntimeError("a map failed during the plain frames")
+    # per-scan latency of the plain form (each scan synchronised)
+    lat = []
+    for m, d, (_, o) in zip(maps[:16], dev[:16], scans[:16]):
+        ctx.event(2)
+        m.insert_point_cloud(d, o, max_range, sync=False)
+        ctx.event(3)
+        lat.append(ctx.elapsed_ms(2, 3))
+    for m in maps:
+        m.close()
+    return {"total_ms": total_ms, "lat": lat}
+
+
+def run_lookups(ctx, bx, scenes, reps=20):
+    coords = scenes.clusters()
+    vals = scenes.cluster_values(coords)
+    q = scenes.lookups(coords)
+    g = bx.VoxelGrid(ctx, 0.05, dtype=np.float32)
+    dc = ctx.to_device(coords)
+    dv = ctx.to_device(vals)
+    g.set_values(dc, dv)
+    assert g.sync() == 0
+    g.clear(bx.CLEAR_MEMORY)
+    ctx.profile_reset()
+    ctx.profile(True)
+    for _ in range(3):
+        g.clear(bx.CLEAR_MEMORY)
+        g.set_values(dc, dv)
+    ctx.profile(False)
+    set_ms = sum(ctx.profile_read(k)[1] / max(ctx.profile_read(k)[0], 1)
+                 for k in ("k_touch", "k_op_resolve", "k_op_apply", "k_op_release"))
+    dq = ctx.to_device(q)
+    out_v = ctx.array(len(q), np.float32)
+    out_f = ctx.array(len(q), np.uint8)
+    g.lookup(dq, out_v, out_f)
+    ctx.sync()
+    ctx.profile_reset()
+    ctx.profile(True)
+    for _ in range(reps):
+        g.lookup(dq, out_v, out_f)
+    ctx.profile(False)
+    n, ms = ctx.profile_read("k_lookup")
+    avg_ms = ms / max(n, 1)
+    found = out_f.numpy()
+    for d in (dc, dv, dq, out_v, out_f):
+        d.free()
+    g.close()
+    nq = len(q)
+    return {
+        "lookups": nq, "lookup_ms": avg_ms, "lookups_per_s": nq / (avg_ms / 1e3),
+        "hit_frac": float(found.mean()), "inserts": len(coords), "insert_ms": set_ms,
+        "inserts_per_s": len(coords) / (set_ms / 1e3),
+    }
+
+
+def cpu_baseline(scans, max_range, budget_s):
+    import oracle
+    t0 = time.perf_counter()
+    pts_done = 0
+    maps_done = 0
+    for pts, origin in scans:
+        m = oracle.ProbMap(0.1)
+        m.insert_point_cloud(pts, origin, max_range)
+        m.close()
+        pts_done += len(pts)
+        maps_done += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return pts_done / dt, maps_done, pts_done, dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--maps", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-lookups", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: every figure is for one MI355X "
+                                   "(see DESIGN.md for how the path would shard)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+
+    max_range = 30.0
+    t0 = time.time()
+    scans = scenes.batch_scans(nmaps=args.maps, max_range=max_range)
+    log(f"generated {len(scans)} scans, {sum(len(p) for p, _ in scans)} points in {time.time() - t0:.1f}s")
+    ctx = bx.Context(0)
+    dev = [ctx.to_device(p) for p, _ in scans]
+
+    b = run_batched(ctx, bx, scans, dev, args.steps, args.warmup, max_range)
+    log(f"batched: {b['total_ms'] / args.steps:.3f} ms/frame")
+    p = None
+    if not args.no_plain:
+        p = run_plain(ctx, bx, scans, dev, args.steps, args.warmup, max_range)
+        log(f"plain: {p['total_ms'] / args.steps:.3f} ms/frame")
+    lk = None
+    if not args.no_lookups:
+        lk = run_lookups(ctx, bx, scenes)
+        log(f"lookups: {lk['lookup_ms']:.3f} ms per 8M")
+
+    npts = b["points"]
+    ms_step = b["total_ms"] / args.steps
+    rays_s = npts / (ms_step / 1e3)
+    # voxel updates per frame: endpoint updates (one per ray) + ray updates
+    upd = b["rays_traced"] + b["ray_updates"]
+    upd_s = upd / (ms_step / 1e3)
+    # algorithmic bytes of one k_scan_rays launch: per ray its 16-byte record
+    # and 4-byte list entry; per updated cell one 4-byte read and one 4-byte
+    # write (every distinct cell once)
+    k_ms = b["kernels_ms"]["k_scan_rays"]
+    alg_bytes = 20 * b["rays_traced"] + 8 * b["ray_updates"]
+    achieved = alg_bytes / (k_ms / 1e3) / 1e9
+    cpu_rate, cpu_maps, cpu_pts, cpu_dt = cpu_baseline(scans, max_range, args.cpu_seconds)
+    lat = np.array(b["lat"])
+    out = {
+        "metric": METRIC,
+        "value": rays_s,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched insertPointCloud: {len(scans)} maps x 1 LiDAR scan each "
+                        "(log-uniform 10k-500k points, box scene), voxel 0.1 m, max range 30 m, "
+                        "one captured graph per frame",
+            "maps": len(scans), "points_per_frame": npts, "voxel": 0.1, "max_range": max_range,
+            "rays_traced_per_frame": b["rays_traced"], "ray_updates_per_frame": b["ray_updates"],
+        },
+        "voxel_updates_per_s": upd_s,
+        "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+        "kernels_ms": b["kernels_ms"],
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_scan_rays",
+            "alg_bytes_per_launch": alg_bytes,
+        },
+        "cpu_baseline": {
+            "value": cpu_rate, "unit": "rays/s", "cores": 1, "kind": "port",
+            "sample": f"the first {cpu_maps} scans of the frame ({cpu_pts} points, {cpu_dt:.1f} s), "
+                      "oracle/bx_oracle.c -O2, one thread",
+        },
+    }
+    if p is not None:
+        plain_rays_s = npts / (p["total_ms"] / args.steps / 1e3)
+        out["plain"] = {"rays_per_s": plain_rays_s, "ms_per_frame": p["total_ms"] / args.steps,
+                        "scan_latency_ms": {"p50": float(np.percentile(p["lat"], 50)),
+                                            "p99": float(np.percentile(p["lat"], 99))}}
+        out["batched_over_plain"] = rays_s / plain_rays_s
+    if lk is not None:
+        lk_bytes = 28 * lk["lookups"]
+        lk_gbs = lk_bytes / (lk["lookup_ms"] / 1e3) / 1e9
+        out["lookup"] = {**lk, "roofline": {"bound": "hbm", "achieved": lk_gbs, "peak": HBM_PEAK_GBS,
+                                            "unit": "GB/s", "frac": lk_gbs / HBM_PEAK_GBS,
+                                            "bytes_per_lookup": 28}}
+    for d in dev:
+        d.free()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -100,6 +100,8 @@ struct bxg_map {
   uint64_t bytes = 0;
   uint64_t known_leaves = 0;
   uint64_t known_roots = 0;
+  uint64_t known_scans = 0;
+  uint64_t max_leaf_growth = 0;  // largest leaf-count increase seen between snapshots
   int4* d_extra = nullptr;  // rays due from addHit/MissPoint
   uint32_t extra_cap = 0;
   std::vector<int4> extra;
@@ -116,6 +118,8 @@ struct bxg_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<bxg_map*> maps;  // maps of the loaded frame
+  std::vector<uint64_t> npoints;
+  std::vector<double> ranges;
   bool loaded = false;
 };
 
@@ -285,14 +289,22 @@ int budget_check(bxg_ctx* c, uint64_t more) {
   return BXG_OK;
 }
 
+// fold a counter snapshot into the host's view of the map
+void take_counters(bxg_map* m) {
+  const uint64_t leaves = std::min<uint64_t>(m->h_ctr->leaf_count, m->leaf_cap);
+  if (leaves > m->known_leaves) m->max_leaf_growth = std::max(m->max_leaf_growth, leaves - m->known_leaves);
+  m->known_leaves = leaves;
+  m->known_roots = m->h_ctr->root_count;
+  m->known_scans = m->h_ctr->scans_applied;
+}
+
 // read the device counters of a map (blocking)
 int read_counters(bxg_map* m) {
   bxg_ctx* c = m->ctx;
   BXG_HIP(c, hipMemcpyAsync(m->h_ctr, m->ctr, sizeof(MapCounters), hipMemcpyDeviceToHost, c->stream));
   BXG_TRY(ctx_sync(c));
   m->ctr_pending = false;
-  m->known_leaves = std::min<uint64_t>(m->h_ctr->leaf_count, m->leaf_cap);
-  m->known_roots = m->h_ctr->root_count;
+  take_counters(m);
   return BXG_OK;
 }
 
@@ -309,8 +321,7 @@ int snapshot_counters(bxg_map* m) {
 void poll_counters(bxg_map* m) {
   if (m->ctr_pending && hipEventQuery(m->ev_ctr) == hipSuccess) {
     m->ctr_pending = false;
-    m->known_leaves = std::min<uint64_t>(m->h_ctr->leaf_count, m->leaf_cap);
-    m->known_roots = m->h_ctr->root_count;
+    take_counters(m);
   }
 }
 
@@ -392,6 +403,20 @@ int grow_roots(bxg_map* m, uint64_t need_roots) {
   c->used += more;
   m->bytes += more;
   return BXG_OK;
+}
+
+// Expected new leaves of an n-point scan with max range R.  Into a map
+// without scans: at most eight leaves a point (long free-space rays of a
+// sparse scan) and at most the leaves of the ball of radius R around the
+// sensor; afterwards twice the largest growth seen so far.
+uint64_t scan_leaf_estimate(const bxg_map* m, uint64_t n, double max_range) {
+  double reach = max_range * m->h.inv_res / (double)(1u << m->h.leaf_bits) + 1.0;
+  if (!(reach < 1e5)) reach = 1e5;
+  const double ball = 4.19 * reach * reach * reach;
+  const double est = std::min<double>((double)n * 8.0, ball) + 4096.0;
+  const uint64_t first = (uint64_t)std::min<double>(est, (double)(1ull << 24));
+  if (m->known_scans == 0) return first;
+  return std::min<uint64_t>(first, 2 * m->max_leaf_growth + 4096);
 }
 
 // keep head-room ahead of a call expected to add up to `leaves` leaves and
@@ -606,7 +631,7 @@ int run_wave(bxg_ctx* c, const bxg_scan_desc* scans, uint32_t nscans, uint32_t e
     // head-room for a scan: about one new leaf per point covers a first
     // LiDAR scan into an empty map (long free-space rays); an overflow is
     // still caught on the device and leaves the cells untouched
-    BXG_TRY(ensure_room(m, std::min<uint64_t>(scans[s].n + 4096, 1u << 24), 1024));
+    BXG_TRY(ensure_room(m, scan_leaf_estimate(m, scans[s].n, scans[s].max_range), 1024));
     if (descs[s].nextra) {
       any_extra = true;
       BXG_TRY(upload_extra(m));
@@ -1623,6 +1648,12 @@ int bxg_graph_set_scans(bxg_graph* g, const bxg_scan_desc* scans, uint32_t nscan
   BXG_TRY(upload(c, g->d_desc, descs.data(), descs.size() * sizeof(ScanDesc)));
   BXG_TRY(upload(c, g->d_hdr, &hdr, sizeof(hdr)));
   g->maps = maps;
+  g->npoints.clear();
+  g->ranges.clear();
+  for (uint32_t s = 0; s < nscans; ++s) {
+    g->npoints.push_back(scans[s].n);
+    g->ranges.push_back(scans[s].max_range);
+  }
   g->loaded = true;
   return BXG_OK;
 }
@@ -1632,7 +1663,8 @@ int bxg_graph_launch(bxg_graph* g) {
   bxg_ctx* c = g->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   if (!g->loaded) return fail(c, BXG_ERR_INVALID_ARG, "no frame loaded");
-  for (bxg_map* m : g->maps) BXG_TRY(ensure_room(m, 1u << 14, 1024));
+  for (size_t s = 0; s < g->maps.size(); ++s)
+    BXG_TRY(ensure_room(g->maps[s], scan_leaf_estimate(g->maps[s], g->npoints[s], g->ranges[s]), 1024));
   BXG_HIP(c, hipGraphLaunch(g->exec, c->stream));
   for (bxg_map* m : g->maps) BXG_TRY(snapshot_counters(m));
   return BXG_OK;

@@ -319,6 +319,7 @@ def test_graph_replays_equal_eager(ctx):
         bx.insert_batch(ctx, [(m, d, o, 30.0) for m, d, (_, o) in zip(emaps, dev, scans)])
         ctx.sync()
         for i, (a, b) in enumerate(zip(gmaps, emaps)):
+            assert a.sync() == 0, f"replay {rep} map {i} reported a device error"
             assert a.digest() == b.digest(), f"replay {rep} map {i}"
             assert a.update_count == b.update_count
     g.close()
