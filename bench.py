@@ -83,8 +83,8 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range):
         bx.insert_batch(ctx, descs)
     ctx.profile(False)
     kern = {}
-    for name in ("k_scan_classify", "k_scan_winners", "k_scan_endpoints", "k_scan_rays",
-                 "k_scan_finalize"):
+    for name in ("k_scan_classify", "k_scan_winners", "k_scan_ray_leaves", "k_scan_endpoints",
+                 "k_scan_rays", "k_scan_finalize"):
         n, ms = ctx.profile_read(name)
         kern[name] = ms / max(n, 1)
     g.close()
@@ -195,6 +195,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-lookups", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--ray-blocks-per-cu", type=int, default=0,
+                    help="override the tuned workgroups per CU of the ray kernels")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run the plain copy kernel over MIB MiB five times (the known-size "
+                         "reference for scaling HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every figure is for one MI355X "
@@ -205,11 +210,19 @@ def main():
     import bonxai_amd as bx
     from bonxai_amd import scenes
 
+    if args.copy_probe:
+        with bx.Context(0) as ctx:
+            ctx.copy_probe(args.copy_probe << 20, reps=5)
+        print(json.dumps({"copy_probe_bytes": args.copy_probe << 20, "reps": 5}))
+        return
+
     max_range = 30.0
     t0 = time.time()
     scans = scenes.batch_scans(nmaps=args.maps, max_range=max_range)
     log(f"generated {len(scans)} scans, {sum(len(p) for p, _ in scans)} points in {time.time() - t0:.1f}s")
     ctx = bx.Context(0)
+    if args.ray_blocks_per_cu:
+        ctx.set_tuning("ray_blocks_per_cu", args.ray_blocks_per_cu)
     dev = [ctx.to_device(p) for p, _ in scans]
 
     b = run_batched(ctx, bx, scans, dev, args.steps, args.warmup, max_range)

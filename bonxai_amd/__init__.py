@@ -104,6 +104,8 @@ ABI = [
     ("bxg_profile_read", _INT, [_P, ctypes.c_char_p, _PU64, _PD]),
     ("bxg_profile_reset", _INT, [_P]),
     ("bxg_set_counting", _INT, [_P, _INT]),
+    ("bxg_ctx_set_tuning", _INT, [_P, ctypes.c_char_p, ctypes.c_int64]),
+    ("bxg_ctx_get_tuning", _INT, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
     ("bxg_map_create", _INT, [_P, _D, _U32, _U32, _U32, _U32, _PPTR]),
     ("bxg_map_destroy", _INT, [_P]),
     ("bxg_map_reserve", _INT, [_P, _U64, _U64]),
@@ -144,6 +146,7 @@ ABI = [
     ("bxg_graph_launch", _INT, [_P]),
     ("bxg_graph_destroy", _INT, [_P]),
     ("bxg_fp64_probe", _INT, [_P, _P, _P, _P, _U64, _P]),
+    ("bxg_copy_probe", _INT, [_P, _P, _P, _U64]),
 ]
 
 _lib = None
@@ -290,8 +293,27 @@ class Context:
         _check(lib().bxg_profile_read(self._h, kernel.encode(), ctypes.byref(n), ctypes.byref(ms)), self)
         return n.value, ms.value
 
+    def set_tuning(self, key, value):
+        _check(lib().bxg_ctx_set_tuning(self._h, key.encode(), int(value)), self, "tuning")
+
+    def get_tuning(self, key):
+        v = ctypes.c_int64()
+        _check(lib().bxg_ctx_get_tuning(self._h, key.encode(), ctypes.byref(v)), self, "tuning")
+        return v.value
+
     def counting(self, on=True):
         _check(lib().bxg_set_counting(self._h, 1 if on else 0), self)
+
+    def copy_probe(self, nbytes, reps=1):
+        """Run the plain copy kernel over nbytes (calibration of byte counters)."""
+        src = self.array(nbytes // 16, np.dtype((np.uint32, 4)))
+        dst = self.array(nbytes // 16, np.dtype((np.uint32, 4)))
+        try:
+            for _ in range(reps):
+                _check(lib().bxg_copy_probe(self._h, src.ptr, dst.ptr, nbytes), self, "copy probe")
+        finally:
+            src.free()
+            dst.free()
 
     def fp64_probe(self, a, b, c):
         a = np.ascontiguousarray(a, np.float64)

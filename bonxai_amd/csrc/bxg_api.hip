@@ -23,6 +23,9 @@ using namespace bxg;
 namespace {
 
 constexpr uint32_t kBlock = 256;
+// scan counters: winner count + two sets of per-region cursors
+constexpr uint32_t kCounterWords = 64;
+static_assert(1 + 4 * kRegions <= kCounterWords, "counter words");
 constexpr int kStageSlots = 8;
 
 struct PinnedSlot {
@@ -80,6 +83,10 @@ struct bxg_ctx {
   hipEvent_t ev[64] = {};
   bool profiling = false;
   bool counting = false;
+  // tuning (bxg_ctx_set_tuning): workgroups per CU of the work-stealing
+  // update pass and leaf-allocation pass over the rays
+  uint32_t ray_blocks_per_cu = 4;
+  uint32_t leaf_blocks_per_cu = 16;
   std::vector<Prof> prof_pending;
   std::vector<hipEvent_t> ev_free;
   std::map<std::string, std::pair<uint64_t, double>> prof_totals;
@@ -438,9 +445,14 @@ int ensure_scan_scratch(bxg_ctx* c, uint64_t points, uint32_t nscans) {
     dfree(c->sc.rec);
     dfree(c->sc.tpos);
     dfree(c->sc.winners);
+    dfree(c->sc.wkey);
     dfree(c->sc.ep_keys);
     dfree(c->sc.ep_first);
+    uint32_t* hist = c->sc.hist;
+    uint32_t* offs = c->sc.offs;
     c->sc = ScanScratch{};
+    c->sc.hist = hist;
+    c->sc.offs = offs;
     const uint64_t cap = std::max<uint64_t>(pow2_at_least(points), 1 << 12);
     if (cap > (1ull << 31)) return fail(c, BXG_ERR_INVALID_ARG, "too many points in one call");
     const uint64_t tcap = cap * 2;
@@ -451,6 +463,15 @@ int ensure_scan_scratch(bxg_ctx* c, uint64_t points, uint32_t nscans) {
     c->sc.tpos = (uint32_t*)p;
     BXG_TRY(dmalloc(c, &p, cap * 4));
     c->sc.winners = (uint32_t*)p;
+    BXG_TRY(dmalloc(c, &p, cap * 4));
+    c->sc.wkey = (uint32_t*)p;
+    if (!c->sc.hist) {
+      BXG_TRY(dmalloc(c, &p, (uint64_t)kMaxBuckets * 4));
+      c->sc.hist = (uint32_t*)p;
+      BXG_TRY(dmalloc(c, &p, (uint64_t)kMaxBuckets * 4));
+      c->sc.offs = (uint32_t*)p;
+      BXG_TRY(fill32(c, c->sc.hist, kMaxBuckets, 0));
+    }
     BXG_TRY(dmalloc(c, &p, tcap * 8));
     c->sc.ep_keys = (uint64_t*)p;
     BXG_TRY(dmalloc(c, &p, tcap * 4));
@@ -595,17 +616,27 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
                          uint64_t grid_points, uint32_t nscans, bool any_extra) {
   const uint32_t gp = nblocks(grid_points);
   const uint32_t gw = std::max<uint32_t>(1, std::min<uint32_t>(gp, 16384));
+  // work-stealing ray kernels: enough blocks to fill every CU, no more
+  // than the rays could use (64 rays per wavefront, 4 wavefronts a block)
+  const uint32_t cus = (uint32_t)std::max(c->cu_count, 1);
+  // at most one lane per point (the winners are fewer)
+  const uint32_t most = (uint32_t)((grid_points + 255) / 256);
+  const uint32_t gr = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->ray_blocks_per_cu, most));
+  const uint32_t gl = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->leaf_blocks_per_cu, most));
   BXG_TRY(launch(c, "k_scan_classify", k_scan_classify, gp, kBlock, hdr, desc, sc));
   BXG_TRY(launch(c, "k_scan_winners", k_scan_winners, gp, kBlock, hdr, desc, sc));
+  BXG_TRY(launch(c, "k_scan_bucket_scan", k_scan_bucket_scan, 1, 1024, hdr, sc));
+  BXG_TRY(launch(c, "k_scan_scatter", k_scan_scatter, gp, kBlock, hdr, sc));
+  BXG_TRY(launch(c, "k_scan_ray_leaves", k_scan_ray_leaves, gl, kBlock, desc, sc));
   if (any_extra)
     BXG_TRY(launch(c, "k_scan_extra_leaves", k_scan_extra_leaves, nscans, kBlock, hdr, desc));
   BXG_TRY(launch(c, "k_scan_endpoints", k_scan_endpoints, gw, kBlock, desc, sc));
   if (c->counting) {
-    BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<true>, gw, kBlock, desc, sc));
+    BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<true>, gr, kBlock, desc, sc));
     if (any_extra)
       BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<true>, nscans, kBlock, hdr, desc));
   } else {
-    BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<false>, gw, kBlock, desc, sc));
+    BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<false>, gr, kBlock, desc, sc));
     if (any_extra)
       BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<false>, nscans, kBlock, hdr, desc));
   }
@@ -618,6 +649,13 @@ int reset_scan_stats(bxg_map* m) {
   bxg_ctx* c = m->ctx;
   const uint64_t z[2] = {0, 0};
   return upload(c, &m->ctr->stat_rays, z, sizeof(z));
+}
+
+// bits of the scan index kept in a winner's bucket
+uint32_t tag_bits_for(uint32_t nscans) {
+  uint32_t b = 0;
+  while (b < kMaxTagBits && (1u << b) < nscans) ++b;
+  return b;
 }
 
 // one wave of scans, each into a different map
@@ -642,7 +680,7 @@ int run_wave(bxg_ctx* c, const bxg_scan_desc* scans, uint32_t nscans, uint32_t e
   // the maps may have grown: descriptors point at device headers, which
   // stay put, so nothing to rebuild
   BXG_TRY(ensure_scan_scratch(c, std::max<uint64_t>(total, 1), nscans));
-  BatchHdr hdr{nscans, 0, total};
+  BatchHdr hdr{nscans, tag_bits_for(nscans), total};
   BXG_TRY(upload(c, c->d_desc, descs.data(), descs.size() * sizeof(ScanDesc)));
   BXG_TRY(upload(c, c->d_hdr, &hdr, sizeof(hdr)));
   BXG_TRY(enqueue_scan_kernels(c, c->d_hdr, c->d_desc, c->sc, total, nscans, any_extra));
@@ -786,7 +824,7 @@ int bxg_ctx_create(int device, uint64_t budget_bytes, bxg_ctx** out) {
   int rc = BXG_OK;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = BXG_ERR_HIP;
   void* p = nullptr;
-  if (rc == BXG_OK) rc = dmalloc(c, &p, 64);
+  if (rc == BXG_OK) rc = dmalloc(c, &p, kCounterWords * 4);
   c->d_counters = (uint32_t*)p;
   if (rc == BXG_OK) rc = dmalloc(c, &p, 64);
   c->d_hdr = (BatchHdr*)p;
@@ -796,7 +834,7 @@ int bxg_ctx_create(int device, uint64_t budget_bytes, bxg_ctx** out) {
     rc = BXG_ERR_HIP;
   for (int i = 0; rc == BXG_OK && i < 64; ++i)
     if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = BXG_ERR_HIP;
-  if (rc == BXG_OK) rc = fill32(c, c->d_counters, 16, 0);
+  if (rc == BXG_OK) rc = fill32(c, c->d_counters, kCounterWords, 0);
   if (rc == BXG_OK) rc = ctx_sync(c);
   if (rc != BXG_OK) {
     bxg_ctx_destroy(c);
@@ -813,6 +851,9 @@ int bxg_ctx_destroy(bxg_ctx* c) {
   dfree(c->sc.rec);
   dfree(c->sc.tpos);
   dfree(c->sc.winners);
+  dfree(c->sc.wkey);
+  dfree(c->sc.hist);
+  dfree(c->sc.offs);
   dfree(c->sc.ep_keys);
   dfree(c->sc.ep_first);
   dfree(c->d_counters);
@@ -954,6 +995,31 @@ int bxg_profile_reset(bxg_ctx* c) {
   c->prof_totals.clear();
   return BXG_OK;
 }
+int bxg_ctx_set_tuning(bxg_ctx* c, const char* key, int64_t value) {
+  if (!c || !key) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  uint32_t* slot = nullptr;
+  if (!std::strcmp(key, "ray_blocks_per_cu")) slot = &c->ray_blocks_per_cu;
+  if (!std::strcmp(key, "leaf_blocks_per_cu")) slot = &c->leaf_blocks_per_cu;
+  if (!slot) return fail(c, BXG_ERR_INVALID_ARG, std::string("unknown tuning key ") + key);
+  if (value < 1 || value > 256) return fail(c, BXG_ERR_INVALID_ARG, std::string(key) + " in 1..256");
+  *slot = (uint32_t)value;
+  return BXG_OK;
+}
+
+int bxg_ctx_get_tuning(bxg_ctx* c, const char* key, int64_t* value) {
+  if (!c || !key || !value) return BXG_ERR_INVALID_ARG;
+  if (!std::strcmp(key, "ray_blocks_per_cu")) {
+    *value = c->ray_blocks_per_cu;
+    return BXG_OK;
+  }
+  if (!std::strcmp(key, "leaf_blocks_per_cu")) {
+    *value = c->leaf_blocks_per_cu;
+    return BXG_OK;
+  }
+  return fail(c, BXG_ERR_INVALID_ARG, std::string("unknown tuning key ") + key);
+}
+
 int bxg_set_counting(bxg_ctx* c, int on) {
   if (!c) return BXG_ERR_INVALID_ARG;
   c->counting = on != 0;
@@ -1582,16 +1648,20 @@ int bxg_graph_create(bxg_ctx* c, uint32_t max_scans, uint64_t max_points, bxg_gr
   g->sc.rec = (int4*)p;
   if (rc == BXG_OK) rc = dmalloc(c, &p, cap * 4), g->sc.tpos = (uint32_t*)p;
   if (rc == BXG_OK) rc = dmalloc(c, &p, cap * 4), g->sc.winners = (uint32_t*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, cap * 4), g->sc.wkey = (uint32_t*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, (uint64_t)kMaxBuckets * 4), g->sc.hist = (uint32_t*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, (uint64_t)kMaxBuckets * 4), g->sc.offs = (uint32_t*)p;
+  if (rc == BXG_OK) rc = fill32(c, g->sc.hist, kMaxBuckets, 0);
   if (rc == BXG_OK) rc = dmalloc(c, &p, tcap * 8), g->sc.ep_keys = (uint64_t*)p;
   if (rc == BXG_OK) rc = dmalloc(c, &p, tcap * 4), g->sc.ep_first = (uint32_t*)p;
-  if (rc == BXG_OK) rc = dmalloc(c, &p, 64), g->d_counters = (uint32_t*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, kCounterWords * 4), g->d_counters = (uint32_t*)p;
   if (rc == BXG_OK) rc = dmalloc(c, &p, (uint64_t)max_scans * sizeof(ScanDesc)), g->d_desc = (ScanDesc*)p;
   if (rc == BXG_OK) rc = dmalloc(c, &p, sizeof(BatchHdr)), g->d_hdr = (BatchHdr*)p;
   g->sc.ep_mask = (uint32_t)(tcap - 1);
   g->sc.counters = g->d_counters;
   if (rc == BXG_OK) rc = fill64(c, g->sc.ep_keys, tcap, kEmptyKey);
   if (rc == BXG_OK) rc = fill32(c, g->sc.ep_first, tcap, kNone);
-  if (rc == BXG_OK) rc = fill32(c, g->d_counters, 16, 0);
+  if (rc == BXG_OK) rc = fill32(c, g->d_counters, kCounterWords, 0);
   if (rc == BXG_OK) {
     BatchHdr hdr{0, 0, 0};
     rc = h2d_sync(c, g->d_hdr, &hdr, sizeof(hdr));
@@ -1644,7 +1714,7 @@ int bxg_graph_set_scans(bxg_graph* g, const bxg_scan_desc* scans, uint32_t nscan
     descs[s].nextra = 0;
     maps.push_back(m);
   }
-  BatchHdr hdr{nscans, 0, total};
+  BatchHdr hdr{nscans, tag_bits_for(nscans), total};
   BXG_TRY(upload(c, g->d_desc, descs.data(), descs.size() * sizeof(ScanDesc)));
   BXG_TRY(upload(c, g->d_hdr, &hdr, sizeof(hdr)));
   g->maps = maps;
@@ -1680,6 +1750,9 @@ int bxg_graph_destroy(bxg_graph* g) {
   dfree(g->sc.rec);
   dfree(g->sc.tpos);
   dfree(g->sc.winners);
+  dfree(g->sc.wkey);
+  dfree(g->sc.hist);
+  dfree(g->sc.offs);
   dfree(g->sc.ep_keys);
   dfree(g->sc.ep_first);
   dfree(g->d_counters);
@@ -1690,6 +1763,16 @@ int bxg_graph_destroy(bxg_graph* g) {
 }
 
 // ------------------------------------------------------------ checks
+
+int bxg_copy_probe(bxg_ctx* c, const void* src, void* dst, uint64_t bytes) {
+  if (!c || (bytes && (!src || !dst)) || (bytes % 16)) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (!bytes) return BXG_OK;
+  const uint64_t n = bytes / 16;
+  BXG_TRY(launch(c, "k_copy_probe", k_copy_probe, (uint32_t)std::min<uint64_t>(nblocks(n), 8192), kBlock,
+                 (const uint4*)src, (uint4*)dst, n));
+  return ctx_sync(c);
+}
 
 int bxg_fp64_probe(bxg_ctx* c, const double* a, const double* b, const double* cc, uint64_t n,
                    double* out) {

@@ -102,9 +102,16 @@ enum : uint32_t {
 // graph can replay any batch up to the capacity it was built for
 struct alignas(16) BatchHdr {
   uint32_t nscans;
-  uint32_t pad;
+  uint32_t tag_bits;  // bits of the scan index in a winner's bucket (0..6)
   uint64_t total;
 };
+
+// Winners are bucketed by (scan index mod 64, direction of the ray) so that
+// the rays one wavefront walks together touch the same cache lines.
+constexpr uint32_t kDirBits = 12;  // 64 x 64 octahedral direction bins
+constexpr uint32_t kMaxTagBits = 6;
+constexpr uint32_t kMaxBuckets = 1u << (kDirBits + kMaxTagBits);
+constexpr uint32_t kRegions = 8;   // winner-list regions, one per XCD
 
 // scratch of one integration call, owned by the context
 struct ScanScratch {
@@ -113,8 +120,12 @@ struct ScanScratch {
   uint64_t* ep_keys;    // endpoint table keys
   uint32_t* ep_first;   // endpoint table: min point index
   uint32_t ep_mask;     // capacity - 1
-  uint32_t* winners;    // flattened point indices of per-voxel winners
-  uint32_t* counters;   // [0] winner count, [1] ray work cursor
+  uint32_t* winners;    // flattened point indices of per-voxel winners, bucketed
+  uint32_t* wkey;       // per point: bucket of a winner, kNone otherwise
+  uint32_t* hist;       // [kMaxBuckets] winners per bucket (zero between calls)
+  uint32_t* offs;       // [kMaxBuckets] bucket write cursors
+  uint32_t* counters;   // [0] winner count, [1..16] leaf-pass region cursors
+                        // and exhausted flags, [17..32] the same for the ray pass
 };
 
 // rec.w flag bits

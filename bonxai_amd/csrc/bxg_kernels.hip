@@ -6,62 +6,73 @@
 //
 // Memory-model notes.  Cross-lane races are resolved with agent-scope atomics
 // (root-slot claims, leaf claims, endpoint first-index claims, mask bits).
+// Reads that only decide whether an atomic is needed are plain loads: a root
+// key or a leaf index never changes once published inside a kernel, so a
+// stale read can only show the EMPTY word, which sends the lane to the CAS.
 // The ray phase writes cell words with plain stores: during that phase every
 // update of a cell is the same function of the same prior word (the miss
 // update guarded by update_id), so concurrent writers store identical
 // values, and a stale read can only return the prior word or the updated
 // word, either of which leads to the same final value.
+//
+// Pointers read out of a map header are cast to the global address space
+// (BXG_G) so loads compile to global_load rather than flat_load.
 #include "bxg_internal.h"
 
+#define BXG_G __attribute__((address_space(1)))
+
 namespace bxg {
+
+template <class T>
+__device__ __forceinline__ T BXG_G* gp(T* p) {
+  return (T BXG_G*)p;
+}
 
 // --------------------------------------------------------------- atomics
 
 __device__ __forceinline__ uint32_t ld32(const uint32_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint64_t ld64(const uint64_t* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_load(gp(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st32(uint32_t* p, uint32_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t cas32(uint32_t* p, uint32_t expect, uint32_t desired) {
-  __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+  __hip_atomic_compare_exchange_strong(gp(p), &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
   return expect;
 }
 __device__ __forceinline__ uint64_t cas64(uint64_t* p, uint64_t expect, uint64_t desired) {
-  __hip_atomic_compare_exchange_strong(p, &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+  __hip_atomic_compare_exchange_strong(gp(p), &expect, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
   return expect;
 }
 __device__ __forceinline__ uint32_t add32(uint32_t* p, uint32_t v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t add64(uint64_t* p, uint64_t v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_fetch_add(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t or32(uint32_t* p, uint32_t v) {
-  return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_fetch_or(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t or64(uint64_t* p, uint64_t v) {
-  return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_fetch_or(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t and64(uint64_t* p, uint64_t v) {
-  return __hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_fetch_and(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint64_t xor64(uint64_t* p, uint64_t v) {
-  return __hip_atomic_fetch_xor(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_fetch_xor(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t min32(uint32_t* p, uint32_t v) {
-  return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_fetch_min(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t max32(uint32_t* p, uint32_t v) {
-  return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return __hip_atomic_fetch_max(gp(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
 
 // Append from the active lanes of a wavefront with one atomic per wavefront;
 // returns this lane's slot (valid only where pred is true).
@@ -72,8 +83,59 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
   uint32_t base = 0;
   if ((int)lane_id() == leader) base = add32(counter, (uint32_t)__popcll(b));
   base = __shfl(base, leader);
-  return base + (uint32_t)__popcll(b & ((1ull << lane_id()) - 1ull));
+  return base + (uint32_t)__popcll(b & lanes_below());
 }
+
+// ------------------------------------------------------------ map view
+
+// A register copy of a map header with global-address-space pointers.
+struct MapView {
+  uint64_t BXG_G* root_keys;
+  uint32_t BXG_G* children;
+  uint64_t BXG_G* masks;
+  uint8_t BXG_G* cells;
+  int4* origin;  // HIP vector types keep the generic address space
+  MapCounters* ctr;
+  double inv_res;
+  uint32_t ib, lb, log2n, isz, lsz, lw, vb, flags;
+  uint32_t root_mask, max_probe, leaf_cap;
+  int32_t miss, hit, cmin, cmax, thr;
+};
+
+__device__ __forceinline__ MapView load_view(const MapDev* p) {
+  const MapDev BXG_G* q = (const MapDev BXG_G*)p;
+  MapView v;
+  v.root_keys = gp(q->root_keys);
+  v.children = gp(q->children);
+  v.masks = gp(q->leaf_masks);
+  v.cells = gp(q->cells);
+  v.origin = q->leaf_origin;
+  v.ctr = q->ctr;
+  v.inv_res = q->inv_res;
+  v.ib = q->inner_bits;
+  v.lb = q->leaf_bits;
+  v.log2n = q->log2n;
+  v.isz = q->inner_size;
+  v.lsz = q->leaf_size;
+  v.lw = q->leaf_words;
+  v.vb = q->value_bytes;
+  v.flags = q->flags;
+  v.root_mask = q->root_mask;
+  v.max_probe = q->max_probe;
+  v.leaf_cap = q->leaf_cap;
+  v.miss = q->opt_miss;
+  v.hit = q->opt_hit;
+  v.cmin = q->opt_cmin;
+  v.cmax = q->opt_cmax;
+  v.thr = q->opt_thr;
+  return v;
+}
+
+// counter words read at kernel start (written by earlier kernels)
+__device__ __forceinline__ uint32_t map_status(const MapView& m) { return *gp(&m.ctr->status); }
+__device__ __forceinline__ int32_t map_cur(const MapView& m) { return *gp(&m.ctr->cur); }
+
+constexpr uint32_t kFatal = kStOutOfPool | kStRootFull | kStCoordRange | kStScratchFull;
 
 // ---------------------------------------------------------- directory
 
@@ -83,12 +145,12 @@ __device__ __forceinline__ uint32_t root_hash(uint64_t key, uint32_t mask) {
 
 // Find-or-insert of a root key (lock-free CAS insert, linear probing).
 // Returns the slot, or -1 (and sets kStRootFull) after max_probe probes.
-__device__ int32_t root_insert(const MapDev& m, uint64_t key) {
+__device__ int32_t root_insert(const MapView& m, uint64_t key) {
   uint32_t h = root_hash(key, m.root_mask);
   for (uint32_t p = 0; p < m.max_probe; ++p) {
-    uint64_t k = ld64(&m.root_keys[h]);
+    uint64_t k = m.root_keys[h];
     if (k == kEmptyKey) {
-      k = cas64(&m.root_keys[h], kEmptyKey, key);
+      k = cas64((uint64_t*)&m.root_keys[h], kEmptyKey, key);
       if (k == kEmptyKey) {
         add32(&m.ctr->root_count, 1u);
         return (int32_t)h;
@@ -102,7 +164,7 @@ __device__ int32_t root_insert(const MapDev& m, uint64_t key) {
 }
 
 // Lookup of a root key in a phase where no insert runs concurrently.
-__device__ __forceinline__ int32_t root_find(const MapDev& m, uint64_t key) {
+__device__ __forceinline__ int32_t root_find(const MapView& m, uint64_t key) {
   uint32_t h = root_hash(key, m.root_mask);
   for (uint32_t p = 0; p <= m.root_mask; ++p) {
     const uint64_t k = m.root_keys[h];
@@ -117,10 +179,10 @@ __device__ __forceinline__ int32_t root_find(const MapDev& m, uint64_t key) {
 // word from EMPTY to PENDING allocates the leaf and publishes its index; all
 // others return at once (they do not need the index before the next kernel).
 // Returns the child word as seen (a leaf index, PENDING, or EMPTY on failure).
-__device__ uint32_t leaf_request(const MapDev& m, uint32_t slot, uint32_t ci, int32_t lx,
+__device__ uint32_t leaf_request(const MapView& m, uint32_t slot, uint32_t ci, int32_t lx,
                                  int32_t ly, int32_t lz) {
-  uint32_t* cp = &m.children[(uint64_t)slot * m.inner_size + ci];
-  uint32_t v = ld32(cp);
+  uint32_t* cp = (uint32_t*)&m.children[(uint64_t)slot * m.isz + ci];
+  uint32_t v = *gp(cp);
   if (v != kEmptyChild) return v;
   v = cas32(cp, kEmptyChild, kPendingChild);
   if (v != kEmptyChild) return v;
@@ -130,19 +192,46 @@ __device__ uint32_t leaf_request(const MapDev& m, uint32_t slot, uint32_t ci, in
     or32(&m.ctr->status, kStOutOfPool);
     return kEmptyChild;
   }
-  m.leaf_origin[idx] = make_int4(lx, ly, lz, (int32_t)slot);
+  m.origin[idx] = make_int4(lx, ly, lz, (int32_t)slot);
   st32(cp, idx);
   return idx;
 }
 
 // leaf index holding coordinate c, or kNone (read-only phases)
-__device__ __forceinline__ uint32_t find_leaf(const MapDev& m, int32_t x, int32_t y, int32_t z) {
+__device__ __forceinline__ uint32_t find_leaf(const MapView& m, int32_t x, int32_t y, int32_t z) {
   uint64_t key;
   if (!pack_root(x, y, z, m.log2n, &key)) return kNone;
   const int32_t s = root_find(m, key);
   if (s < 0) return kNone;
-  const uint32_t c = m.children[(uint64_t)s * m.inner_size + inner_index(x, y, z, m.leaf_bits, m.inner_bits)];
+  const uint32_t c = m.children[(uint64_t)s * m.isz + inner_index(x, y, z, m.lb, m.ib)];
   return c >= kPendingChild ? kNone : c;
+}
+
+// The root slot of the last root a walk visited.
+struct RootCache {
+  int32_t rx, ry, rz, slot;
+  bool valid;
+  __device__ __forceinline__ void reset() { valid = false; }
+};
+
+// slot of the root holding (x, y, z): looked up (kInsert false) or created
+// (kInsert true), cached while the walk stays in the root
+template <bool kInsert>
+__device__ __forceinline__ int32_t root_slot(const MapView& m, RootCache& rc, int32_t x, int32_t y,
+                                             int32_t z) {
+  const int32_t rmask = ~((1 << m.log2n) - 1);
+  const int32_t rx = x & rmask, ry = y & rmask, rz = z & rmask;
+  if (!rc.valid || rx != rc.rx || ry != rc.ry || rz != rc.rz) {
+    uint64_t key;
+    int32_t s = -1;
+    if (pack_root(x, y, z, m.log2n, &key)) s = kInsert ? root_insert(m, key) : root_find(m, key);
+    rc.rx = rx;
+    rc.ry = ry;
+    rc.rz = rz;
+    rc.slot = s;
+    rc.valid = true;
+  }
+  return rc.slot;
 }
 
 // ------------------------------------------------------- cell update
@@ -171,13 +260,13 @@ __device__ __forceinline__ uint32_t cell_miss(uint32_t w, int32_t miss, int32_t 
 
 // Integer 3-D Bresenham walk of RayIterator (probabilistic_map.hpp:162-203):
 // cell 0 is the origin voxel, then max|d|-1 further cells; the end voxel is
-// not part of the ray and a zero-length ray is empty.
+// not part of the ray and a zero-length ray is empty.  (cx, cy, cz) is cell k.
 struct RayWalk {
   int32_t cx, cy, cz;
   int32_t sx, sy, sz;
   int32_t dx, dy, dz;
   int32_t ex, ey, ez;
-  int32_t mx;
+  int32_t mx, k;
   __device__ __forceinline__ void init(int32_t ox, int32_t oy, int32_t oz, int32_t tx, int32_t ty,
                                        int32_t tz) {
     cx = ox;
@@ -195,8 +284,12 @@ struct RayWalk {
     mx = dx > dy ? dx : dy;
     mx = mx > dz ? mx : dz;
     ex = ey = ez = 0;
+    k = 0;
   }
-  __device__ __forceinline__ void step() {
+  __device__ __forceinline__ bool done() const { return k >= mx; }
+  // move to cell k + 1
+  __device__ __forceinline__ void advance() {
+    ++k;
     ex += dx;
     ey += dy;
     ez += dz;
@@ -214,6 +307,92 @@ struct RayWalk {
     }
   }
 };
+
+constexpr int kRun = 8;  // cells of one leaf handled per step of a ray
+
+// Advance a walk through the cells of its current leaf (at most kRun of
+// them), recording their leaf-local indices; returns how many.
+__device__ __forceinline__ uint32_t collect_run(const MapView& m, RayWalk& w, uint32_t li[kRun]) {
+  const int32_t lmask = ~((1 << m.lb) - 1);
+  const int32_t lx = w.cx & lmask, ly = w.cy & lmask, lz = w.cz & lmask;
+  uint32_t n = 0;
+#pragma unroll
+  for (int j = 0; j < kRun; ++j) {
+    if (!w.done() && (w.cx & lmask) == lx && (w.cy & lmask) == ly && (w.cz & lmask) == lz) {
+      li[j] = leaf_index(w.cx, w.cy, w.cz, m.lb);
+      n = j + 1;
+      w.advance();
+    }
+  }
+  return n;
+}
+
+// One leaf of a ray, allocation pass: create the root and the leaf of the
+// current cell, then move past the ray's cells in that leaf (kRun at a time).
+__device__ __forceinline__ void leaves_step(const MapView& m, RayWalk& w, RootCache& rc) {
+  const int32_t lmask = ~((1 << m.lb) - 1);
+  const int32_t x = w.cx, y = w.cy, z = w.cz;
+  const int32_t slot = root_slot<true>(m, rc, x, y, z);
+  if (slot >= 0)
+    leaf_request(m, (uint32_t)slot, inner_index(x, y, z, m.lb, m.ib), x & lmask, y & lmask, z & lmask);
+  uint32_t li[kRun];
+  while (collect_run(m, w, li) == (uint32_t)kRun && !w.done() && (w.cx & lmask) == (x & lmask) &&
+         (w.cy & lmask) == (y & lmask) && (w.cz & lmask) == (z & lmask)) {
+  }
+}
+
+// One leaf of a ray, update pass: the cells of the ray in the current leaf
+// (up to kRun) are loaded together, then each takes the miss update unless
+// its update_id already equals cur.  Returns the cells written (counted
+// form: exact, by CAS).
+// leaf index of the walk's current cell (read-only phases)
+__device__ __forceinline__ uint32_t walk_leaf(const MapView& m, const RayWalk& w, RootCache& rc) {
+  const int32_t slot = root_slot<false>(m, rc, w.cx, w.cy, w.cz);
+  if (slot < 0) return kNone;
+  const uint32_t c = m.children[(uint64_t)slot * m.isz + inner_index(w.cx, w.cy, w.cz, m.lb, m.ib)];
+  return c >= kPendingChild ? kNone : c;
+}
+
+// `leaf` is the leaf of the walk's current cell on entry; on return it is the
+// leaf of the next run (looked up before this run's cells are loaded, so the
+// two memory round trips overlap).
+template <bool kCount>
+__device__ __forceinline__ uint32_t apply_step(const MapView& m, RayWalk& w, RootCache& rc,
+                                               int32_t cur, uint32_t& next_leaf) {
+  const uint32_t leaf = next_leaf;
+  uint32_t li[kRun];
+  const uint32_t n = collect_run(m, w, li);
+  next_leaf = w.done() ? kNone : walk_leaf(m, w, rc);
+  if (leaf == kNone) return 0;
+  uint32_t BXG_G* base = (uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz;
+  uint32_t v[kRun];
+#pragma unroll
+  for (int j = 0; j < kRun; ++j)
+    if ((uint32_t)j < n) v[j] = base[li[j]];
+  uint32_t nupd = 0;
+#pragma unroll
+  for (int j = 0; j < kRun; ++j) {
+    if ((uint32_t)j < n && cell_id(v[j]) != cur) {
+      uint32_t old = v[j];
+      if (kCount) {
+        for (;;) {
+          const uint32_t seen = cas32((uint32_t*)&base[li[j]], old, cell_miss(old, m.miss, m.cmin, cur));
+          if (seen == old) {
+            ++nupd;
+            break;
+          }
+          old = seen;
+          if (cell_id(old) == cur) break;
+        }
+      } else {
+        base[li[j]] = cell_miss(old, m.miss, m.cmin, cur);
+      }
+      if (old == 0)
+        or64((uint64_t*)&m.masks[(uint64_t)leaf * m.lw + (li[j] >> 6)], 1ull << (li[j] & 63));
+    }
+  }
+  return nupd;
+}
 
 // --------------------------------------------------------- scan phases
 
@@ -245,16 +424,16 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
   if (gi >= hdr->total) return;
   const uint32_t s = find_scan(sd, hdr->nscans, gi);
   const ScanDesc& d = sd[s];
-  const MapDev& m = *d.map;
+  const MapView m = load_view(d.map);
   const uint64_t i = gi - d.pt_off;
   double tx, ty, tz;
   if (d.flags & kScanF64) {
-    const double* p = (const double*)d.points + 3 * i;
+    const double BXG_G* p = gp((const double*)d.points) + 3 * i;
     tx = p[0];
     ty = p[1];
     tz = p[2];
   } else {
-    const float* p = (const float*)d.points + 3 * i;
+    const float BXG_G* p = gp((const float*)d.points) + 3 * i;
     tx = (double)p[0];
     ty = (double)p[1];
     tz = (double)p[2];
@@ -290,21 +469,21 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
     sc.rec[gi] = make_int4(ex, ey, ez, (int32_t)(s << 16));
     return;
   }
-  const uint32_t ci = inner_index(ex, ey, ez, m.leaf_bits, m.inner_bits);
-  const uint32_t li = leaf_index(ex, ey, ez, m.leaf_bits);
-  const int32_t lmask = ~((1 << m.leaf_bits) - 1);
+  const uint32_t ci = inner_index(ex, ey, ez, m.lb, m.ib);
+  const uint32_t li = leaf_index(ex, ey, ez, m.lb);
+  const int32_t lmask = ~((1 << m.lb) - 1);
   const uint32_t leaf = leaf_request(m, (uint32_t)slot, ci, ex & lmask, ey & lmask, ez & lmask);
   bool marked = true;  // a leaf created in this scan has all cells zero: id 0
   if (leaf < kPendingChild) {
-    const uint32_t w = ld32((const uint32_t*)m.cells + (uint64_t)leaf * m.leaf_size + li);
-    marked = cell_id(w) != m.ctr->cur;
+    const uint32_t w = ((const uint32_t BXG_G*)m.cells)[(uint64_t)leaf * m.lsz + li];
+    marked = cell_id(w) != map_cur(m);
   }
   if (marked) {
     const uint64_t key = ep_key(d.tag, (uint32_t)slot, ci, li);
     uint32_t h = (uint32_t)mix64(key) & sc.ep_mask;
     uint32_t tp = kNone;
     for (uint32_t p = 0; p <= sc.ep_mask; ++p) {
-      uint64_t k = ld64(&sc.ep_keys[h]);
+      uint64_t k = gp(sc.ep_keys)[h];
       if (k == kEmptyKey) {
         k = cas64(&sc.ep_keys[h], kEmptyKey, key);
         if (k == kEmptyKey) k = key;
@@ -326,123 +505,254 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
   sc.rec[gi] = make_int4(ex, ey, ez, (int32_t)(flags | (s << 16)));
 }
 
-// every leaf on Ray(o, e) created (no cell touched)
-__device__ void ray_leaves(const MapDev& m, int32_t ox, int32_t oy, int32_t oz, int32_t tx,
-                           int32_t ty, int32_t tz) {
-  RayWalk w;
-  w.init(ox, oy, oz, tx, ty, tz);
-  const int32_t lmask = ~((1 << m.leaf_bits) - 1);
-  const int32_t rmask = ~((1 << m.log2n) - 1);
-  int32_t plx = w.cx ^ 1, ply = 0, plz = 0;  // no previous leaf
-  int32_t prx = w.cx ^ 1, pry = 0, prz = 0;
-  int32_t slot = -1;
-  for (int32_t k = 0; k < w.mx; ++k) {
-    if (k) w.step();
-    const int32_t lx = w.cx & lmask, ly = w.cy & lmask, lz = w.cz & lmask;
-    if (lx != plx || ly != ply || lz != plz) {
-      plx = lx;
-      ply = ly;
-      plz = lz;
-      const int32_t rx = w.cx & rmask, ry = w.cy & rmask, rz = w.cz & rmask;
-      if (rx != prx || ry != pry || rz != prz) {
-        prx = rx;
-        pry = ry;
-        prz = rz;
-        uint64_t key;
-        pack_root(w.cx, w.cy, w.cz, m.log2n, &key);
-        slot = root_insert(m, key);
-      }
-      if (slot < 0) return;
-      leaf_request(m, (uint32_t)slot, inner_index(w.cx, w.cy, w.cz, m.leaf_bits, m.inner_bits), lx,
-                   ly, lz);
-    }
+// Lanes of a wavefront that share a key form one group: one atomic add of
+// the group size per group; returns this lane's rank-offset (base + rank).
+__device__ __forceinline__ uint32_t wave_add_by_key(uint32_t* table, uint32_t key, bool pred) {
+  uint64_t pending = __ballot(pred);
+  uint32_t pos = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const uint32_t lk = (uint32_t)__shfl((int)key, leader);
+    const uint64_t same = __ballot(pred && key == lk);
+    uint32_t base = 0;
+    if ((int)lane_id() == leader) base = add32(&table[lk], (uint32_t)__popcll(same));
+    base = (uint32_t)__shfl((int)base, leader);
+    if ((same >> lane_id()) & 1ull) pos = base + (uint32_t)__popcll(same & lanes_below());
+    pending &= ~same;
   }
+  return pos;
 }
 
-// The miss update of every cell of Ray(o, e) whose update_id differs from
-// cur; returns the number of cells written when counting (exact CAS form).
-template <bool kCount>
-__device__ uint32_t ray_apply(const MapDev& m, int32_t ox, int32_t oy, int32_t oz, int32_t tx,
-                              int32_t ty, int32_t tz, int32_t cur) {
-  const int32_t miss = m.opt_miss, cmin = m.opt_cmin;
-  uint32_t* cells = (uint32_t*)m.cells;
-  const int32_t lmask = ~((1 << m.leaf_bits) - 1);
-  RayWalk w;
-  w.init(ox, oy, oz, tx, ty, tz);
-  int32_t plx = w.cx ^ 1, ply = 0, plz = 0;
-  uint32_t leaf = 0;
-  uint32_t nupd = 0;
-  for (int32_t k = 0; k < w.mx; ++k) {
-    if (k) w.step();
-    const int32_t lx = w.cx & lmask, ly = w.cy & lmask, lz = w.cz & lmask;
-    if (lx != plx || ly != ply || lz != plz) {
-      plx = lx;
-      ply = ly;
-      plz = lz;
-      leaf = find_leaf(m, w.cx, w.cy, w.cz);
-    }
-    if (leaf == kNone) continue;
-    const uint32_t li = leaf_index(w.cx, w.cy, w.cz, m.leaf_bits);
-    uint32_t* cp = cells + (uint64_t)leaf * m.leaf_size + li;
-    uint32_t v = *cp;
-    if (cell_id(v) != cur) {
-      if (kCount) {
-        for (;;) {
-          const uint32_t old = cas32(cp, v, cell_miss(v, miss, cmin, cur));
-          if (old == v) {
-            ++nupd;
-            break;
-          }
-          v = old;
-          if (cell_id(v) == cur) break;
-        }
-      } else {
-        *cp = cell_miss(v, miss, cmin, cur);
-      }
-      if (v == 0) or64(&m.leaf_masks[(uint64_t)leaf * m.leaf_words + (li >> 6)], 1ull << (li & 63));
-    }
+// octahedral direction bin of (dx, dy, dz), 64 x 64 bins in Morton order
+__device__ __forceinline__ uint32_t dir_bin(int32_t dx, int32_t dy, int32_t dz) {
+  const float ax = fabsf((float)dx), ay = fabsf((float)dy), az = fabsf((float)dz);
+  const float s = ax + ay + az;
+  if (s == 0.f) return 0;
+  float u = (float)dx / s, v = (float)dy / s;
+  if (dz < 0) {
+    const float uu = (1.f - fabsf(v)) * (u >= 0.f ? 1.f : -1.f);
+    const float vv = (1.f - fabsf(u)) * (v >= 0.f ? 1.f : -1.f);
+    u = uu;
+    v = vv;
   }
-  return nupd;
+  uint32_t iu = (uint32_t)((u + 1.f) * 32.f), iv = (uint32_t)((v + 1.f) * 32.f);
+  iu = iu > 63u ? 63u : iu;
+  iv = iv > 63u ? 63u : iv;
+  uint32_t key = 0;
+#pragma unroll
+  for (int b = 0; b < 6; ++b) key |= (((iu >> b) & 1u) << (2 * b)) | (((iv >> b) & 1u) << (2 * b + 1));
+  return key;
 }
 
 // Phase 2 (per point): the lowest-index point of each claimed voxel is its
 // winner (probabilistic_map.cpp:30-54 lets only the first point of a voxel
-// through the update_id test); winners are listed, and every leaf on the
-// winner's ray is created before any cell is written.
+// through the update_id test); winners are counted per bucket.
 __global__ void __launch_bounds__(256) k_scan_winners(const BatchHdr* __restrict__ hdr,
                                                       const ScanDesc* __restrict__ sd,
                                                       ScanScratch sc) {
   const uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   bool win = false;
-  int4 r = make_int4(0, 0, 0, 0);
+  uint32_t key = 0;
   if (gi < hdr->total) {
-    r = sc.rec[gi];
-    if ((r.w & kRecMarked) && sc.ep_first[sc.tpos[gi]] == (uint32_t)gi) win = true;
+    const int4 r = sc.rec[gi];
+    if ((r.w & kRecMarked) && sc.ep_first[sc.tpos[gi]] == (uint32_t)gi) {
+      win = true;
+      const uint32_t s = (uint32_t)r.w >> 16;
+      const ScanDesc& d = sd[s];
+      const uint32_t tag = s & ((1u << hdr->tag_bits) - 1u);
+      key = (tag << kDirBits) | dir_bin(r.x - d.ox, r.y - d.oy, r.z - d.oz);
+    }
+    sc.wkey[gi] = win ? key : kNone;
   }
-  const uint32_t pos = wave_append(&sc.counters[0], win);
-  if (!win) return;
-  sc.winners[pos] = (uint32_t)gi;
-  const ScanDesc& d = sd[(uint32_t)r.w >> 16];
-  if (d.flags & kScanNoRays) return;
-  if (r.x == d.ox && r.y == d.oy && r.z == d.oz) return;
-  ray_leaves(*d.map, d.ox, d.oy, d.oz, r.x, r.y, r.z);
+  wave_add_by_key(sc.hist, key, win);
 }
 
-// Phase 2b (one block per scan): leaves of the rays still due from
+// Phase 2s (one block): exclusive scan of the bucket counts into write
+// cursors; the winner count goes to counters[0] and the counts are zeroed.
+__global__ void __launch_bounds__(1024) k_scan_bucket_scan(const BatchHdr* __restrict__ hdr,
+                                                           ScanScratch sc) {
+  __shared__ uint32_t part[1024];
+  const uint32_t nb = 1u << (kDirBits + hdr->tag_bits);
+  const uint32_t per = (nb + 1023u) / 1024u;
+  const uint32_t t = threadIdx.x;
+  const uint32_t lo = t * per, hi = lo + per < nb ? lo + per : nb;
+  uint32_t sum = 0;
+  for (uint32_t i = lo; i < hi; ++i) sum += sc.hist[i];
+  part[t] = sum;
+  __syncthreads();
+  // Hillis-Steele inclusive scan of the 1024 partial sums
+  for (uint32_t off = 1; off < 1024u; off <<= 1) {
+    const uint32_t v = t >= off ? part[t - off] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t run = part[t] - sum;
+  for (uint32_t i = lo; i < hi; ++i) {
+    const uint32_t c = sc.hist[i];
+    sc.offs[i] = run;
+    sc.hist[i] = 0;
+    run += c;
+  }
+  if (t == 1023) sc.counters[0] = part[1023];
+}
+
+// Phase 2t (per point): winners written into their buckets.
+__global__ void __launch_bounds__(256) k_scan_scatter(const BatchHdr* __restrict__ hdr,
+                                                      ScanScratch sc) {
+  const uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t key = kNone;
+  if (gi < hdr->total) key = sc.wkey[gi];
+  const bool win = key != kNone;
+  const uint32_t pos = wave_add_by_key(sc.offs, win ? key : 0u, win);
+  if (win) sc.winners[pos] = (uint32_t)gi;
+}
+
+// Start the ray of winner `idx`; false if it has no cells to walk (origin
+// voxel endpoint, endpoint-only call, or a map whose call failed).
+__device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const ScanScratch& sc,
+                                          uint32_t idx, MapView& m, RayWalk& w, RootCache& rc,
+                                          int32_t& cur, bool check_status) {
+  const uint32_t gi = sc.winners[idx];
+  const int4 r = sc.rec[gi];
+  const ScanDesc& d = sd[(uint32_t)r.w >> 16];
+  if (d.flags & kScanNoRays) return false;
+  if (r.x == d.ox && r.y == d.oy && r.z == d.oz) return false;
+  m = load_view(d.map);
+  if (check_status && (map_status(m) & kFatal)) return false;
+  cur = map_cur(m);
+  w.init(d.ox, d.oy, d.oz, r.x, r.y, r.z);
+  rc.reset();
+  return true;
+}
+
+constexpr uint32_t kChunk = 256;  // most rays a wavefront takes per cursor atomic
+
+// Work-stealing walk over the winner list.  The list is split into kRegions
+// ranges; a wavefront starts in the range of its XCD (workgroups are
+// dispatched round-robin over the XCDs) and takes kChunk rays per atomic on
+// the range's cursor; every lane works on its own ray one leaf per step and
+// a lane whose ray ends takes the next ray of the wavefront's chunk.  A
+// range found empty is flagged so that idle wavefronts stop touching its
+// cursor.  kMode 0: leaf allocation pass; 1: update pass; 2: counted update.
+// cursors[0..kRegions) are the cursors, cursors[kRegions..2*kRegions) the
+// exhausted flags.
+template <int kMode>
+__device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch& sc,
+                             uint32_t* cursors) {
+  const uint32_t nwin = sc.counters[0];
+  const uint32_t lane = lane_id();
+  uint32_t chunk_pos = 0, chunk_end = 0;
+  // small lists are one region: splitting them only multiplies cursor traffic
+  const uint32_t nreg = nwin >= kRegions * 8 * kChunk ? kRegions : 1u;
+  // chunk size: kChunk rays when there is work for every wavefront, down
+  // to one ray a lane on small lists
+  uint32_t chunk = (nwin / (gridDim.x * (blockDim.x / 64u))) & ~63u;
+  chunk = chunk < 64u ? 64u : (chunk > kChunk ? kChunk : chunk);
+  uint32_t region = blockIdx.x % nreg, tried = 0;
+  bool exhausted = false;
+  bool have = false;
+  MapView m;
+  RayWalk w;
+  RootCache rc;
+  int32_t cur = 0;
+  uint32_t nupd = 0;
+  uint32_t next_leaf = kNone;
+  // counted form: per-lane totals, flushed when the lane's map changes
+  MapCounters* cmap = nullptr;
+  uint64_t c_upd = 0, c_rays = 0;
+  for (;;) {
+    // hand out rays to the lanes without one (wave-uniform control flow)
+    for (;;) {
+      const uint64_t need = __ballot(!have);
+      if (need == 0 || exhausted) break;
+      if (chunk_pos == chunk_end) {
+        const uint32_t lo = (uint32_t)((uint64_t)nwin * region / nreg);
+        const uint32_t hi = (uint32_t)((uint64_t)nwin * (region + 1) / nreg);
+        uint32_t base = hi - lo;
+        // plain reads first: an exhausted range costs no atomic
+        if (lane == 0 && !*gp(&cursors[kRegions + region]) && *gp(&cursors[region]) < hi - lo)
+          base = add32(&cursors[region], chunk);
+        base = (uint32_t)__shfl((int)base, 0);
+        if (base >= hi - lo) {
+          if (lane == 0) *gp(&cursors[kRegions + region]) = 1u;
+          if (++tried == nreg) {
+            exhausted = true;
+            break;
+          }
+          region = (region + 1) % nreg;
+          continue;
+        }
+        chunk_pos = lo + base;
+        chunk_end = chunk_pos + chunk < hi ? chunk_pos + chunk : hi;
+        continue;
+      }
+      const uint32_t avail = chunk_end - chunk_pos;
+      const uint32_t want = (uint32_t)__popcll(need);
+      const uint32_t take = want < avail ? want : avail;
+      const uint32_t rank = (uint32_t)__popcll(need & lanes_below());
+      if (!have && rank < take) {
+        have = start_ray(sd, sc, chunk_pos + rank, m, w, rc, cur, kMode != 0);
+        nupd = 0;
+        if (kMode != 0 && have) next_leaf = walk_leaf(m, w, rc);
+        if (kMode == 2 && have && m.ctr != cmap) {
+          if (cmap) {
+            add64(&cmap->stat_ray_updates, c_upd);
+            add64(&cmap->stat_rays, c_rays);
+          }
+          cmap = m.ctr;
+          c_upd = c_rays = 0;
+        }
+      }
+      chunk_pos += take;
+    }
+    if (__ballot(have) == 0) break;
+    if (have) {
+      if (kMode == 0) {
+        leaves_step(m, w, rc);
+      } else {
+        nupd += apply_step<kMode == 2>(m, w, rc, cur, next_leaf);
+      }
+      if (w.done()) {
+        have = false;
+        if (kMode == 2) {
+          c_upd += nupd;
+          c_rays += 1;
+        }
+      }
+    }
+  }
+  if (kMode == 2 && cmap) {
+    add64(&cmap->stat_ray_updates, c_upd);
+    add64(&cmap->stat_rays, c_rays);
+  }
+}
+
+// Phase 2b: every leaf on every winner's ray is created before any cell is
+// written (so a pool overflow leaves the map's cells untouched).
+__global__ void __launch_bounds__(256) k_scan_ray_leaves(const ScanDesc* __restrict__ sd,
+                                                         ScanScratch sc) {
+  walk_winners<0>(sd, sc, &sc.counters[1]);
+}
+
+// Phase 2c (one block per scan): leaves of the rays still due from
 // addHitPoint / addMissPoint calls made since the map's last insert.
 __global__ void __launch_bounds__(256) k_scan_extra_leaves(const BatchHdr* __restrict__ hdr,
                                                            const ScanDesc* __restrict__ sd) {
   if (blockIdx.x >= hdr->nscans) return;
   const ScanDesc& d = sd[blockIdx.x];
   if (d.flags & kScanNoRays) return;
+  const MapView m = load_view(d.map);
   for (uint32_t j = threadIdx.x; j < d.nextra; j += blockDim.x) {
     const int4 e = d.extra[j];
     if (e.x == d.ox && e.y == d.oy && e.z == d.oz) continue;
-    ray_leaves(*d.map, d.ox, d.oy, d.oz, e.x, e.y, e.z);
+    RayWalk w;
+    RootCache rc;
+    rc.reset();
+    w.init(d.ox, d.oy, d.oz, e.x, e.y, e.z);
+    while (!w.done()) leaves_step(m, w, rc);
   }
 }
-
-constexpr uint32_t kFatal = kStOutOfPool | kStRootFull | kStCoordRange | kStScratchFull;
 
 // Phase 3 (per winner): endpoint hit/miss update (probabilistic_map.cpp:30-54)
 // and release of the endpoint-table slot.  One writer per endpoint cell.
@@ -456,7 +766,7 @@ __global__ void __launch_bounds__(256) k_scan_endpoints(const ScanDesc* __restri
   for (uint32_t base = start - (start & 63u); base < nwin; base += stride) {
     const uint32_t idx = base + (start & 63u);
     uint32_t updated = 0;
-    const MapDev* mp = nullptr;
+    MapCounters* ctr = nullptr;
     if (idx < nwin) {
       const uint32_t gi = sc.winners[idx];
       const int4 r = sc.rec[gi];
@@ -464,55 +774,38 @@ __global__ void __launch_bounds__(256) k_scan_endpoints(const ScanDesc* __restri
       sc.ep_keys[t] = kEmptyKey;
       sc.ep_first[t] = kNone;
       const ScanDesc& d = sd[(uint32_t)r.w >> 16];
-      const MapDev& m = *d.map;
-      const uint32_t leaf = (m.ctr->status & kFatal) ? kNone : find_leaf(m, r.x, r.y, r.z);
+      const MapView m = load_view(d.map);
+      const uint32_t leaf = (map_status(m) & kFatal) ? kNone : find_leaf(m, r.x, r.y, r.z);
       if (leaf != kNone) {
-        const uint32_t li = leaf_index(r.x, r.y, r.z, m.leaf_bits);
-        uint32_t* cell = (uint32_t*)m.cells + (uint64_t)leaf * m.leaf_size + li;
+        const uint32_t li = leaf_index(r.x, r.y, r.z, m.lb);
+        uint32_t BXG_G* cell = (uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz + li;
         const uint32_t w = *cell;
-        const int32_t cur = m.ctr->cur;
-        *cell = (r.w & kRecHit) ? cell_hit(w, m.opt_hit, m.opt_cmax, cur)
-                                : cell_miss(w, m.opt_miss, m.opt_cmin, cur);
-        if (w == 0)
-          or64(&m.leaf_masks[(uint64_t)leaf * m.leaf_words + (li >> 6)], 1ull << (li & 63));
+        const int32_t cur = map_cur(m);
+        *cell = (r.w & kRecHit) ? cell_hit(w, m.hit, m.cmax, cur) : cell_miss(w, m.miss, m.cmin, cur);
+        if (w == 0) or64((uint64_t*)&m.masks[(uint64_t)leaf * m.lw + (li >> 6)], 1ull << (li & 63));
         updated = 1;
-        mp = &m;
+        ctr = m.ctr;
       }
     }
     // one counter update per distinct map among the wavefront's lanes
     uint64_t pending = __ballot(updated);
     while (pending) {
       const int leader = __ffsll((unsigned long long)pending) - 1;
-      const MapDev* lm = (const MapDev*)__shfl((unsigned long long)mp, leader);
-      const uint64_t same = __ballot(updated && mp == lm);
-      if ((int)lane_id() == leader) add64(&lm->ctr->stat_endpoint_updates, (uint64_t)__popcll(same));
+      MapCounters* lc = (MapCounters*)__shfl((unsigned long long)ctr, leader);
+      const uint64_t same = __ballot(updated && ctr == lc);
+      if ((int)lane_id() == leader) add64(&lc->stat_endpoint_updates, (uint64_t)__popcll(same));
       pending &= ~same;
     }
   }
 }
 
-// Phase 4 (per winner): the ray clear of updateFreeCells
-// (probabilistic_map.cpp:77-101): every cell of Ray(origin, endpoint) takes
-// the miss update unless its update_id already equals the scan's id.
+// Phase 4: the ray clear of updateFreeCells (probabilistic_map.cpp:77-101):
+// every cell of Ray(origin, endpoint) takes the miss update unless its
+// update_id already equals the scan's id.
 template <bool kCount>
 __global__ void __launch_bounds__(256) k_scan_rays(const ScanDesc* __restrict__ sd,
                                                    ScanScratch sc) {
-  const uint32_t nwin = sc.counters[0];
-  for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < nwin;
-       idx += gridDim.x * blockDim.x) {
-    const uint32_t gi = sc.winners[idx];
-    const int4 r = sc.rec[gi];
-    const ScanDesc& d = sd[(uint32_t)r.w >> 16];
-    const MapDev& m = *d.map;
-    if (d.flags & kScanNoRays) continue;
-    if (m.ctr->status & kFatal) continue;
-    if (r.x == d.ox && r.y == d.oy && r.z == d.oz) continue;
-    const uint32_t n = ray_apply<kCount>(m, d.ox, d.oy, d.oz, r.x, r.y, r.z, m.ctr->cur);
-    if (kCount) {
-      add64(&m.ctr->stat_ray_updates, n);
-      add64(&m.ctr->stat_rays, 1);
-    }
-  }
+  walk_winners<kCount ? 2 : 1>(sd, sc, &sc.counters[1 + 2 * kRegions]);
 }
 
 // Phase 4b (one block per scan): rays due from addHitPoint / addMissPoint.
@@ -522,12 +815,19 @@ __global__ void __launch_bounds__(256) k_scan_extra_rays(const BatchHdr* __restr
   if (blockIdx.x >= hdr->nscans) return;
   const ScanDesc& d = sd[blockIdx.x];
   if (d.flags & kScanNoRays) return;
-  const MapDev& m = *d.map;
-  if (m.ctr->status & kFatal) return;
+  const MapView m = load_view(d.map);
+  if (map_status(m) & kFatal) return;
+  const int32_t cur = map_cur(m);
   for (uint32_t j = threadIdx.x; j < d.nextra; j += blockDim.x) {
     const int4 e = d.extra[j];
     if (e.x == d.ox && e.y == d.oy && e.z == d.oz) continue;
-    const uint32_t n = ray_apply<kCount>(m, d.ox, d.oy, d.oz, e.x, e.y, e.z, m.ctr->cur);
+    RayWalk w;
+    RootCache rc;
+    rc.reset();
+    w.init(d.ox, d.oy, d.oz, e.x, e.y, e.z);
+    uint32_t n = 0;
+    uint32_t next_leaf = walk_leaf(m, w, rc);
+    while (!w.done()) n += apply_step<kCount>(m, w, rc, cur, next_leaf);
     if (kCount) {
       add64(&m.ctr->stat_ray_updates, n);
       add64(&m.ctr->stat_rays, 1);
@@ -541,13 +841,13 @@ __global__ void k_scan_finalize(const BatchHdr* __restrict__ hdr, const ScanDesc
                                 ScanScratch sc) {
   for (uint32_t s = threadIdx.x; s < hdr->nscans; s += blockDim.x) {
     if (sd[s].flags & kScanNoRays) continue;
-    const MapDev& m = *sd[s].map;
-    if (m.ctr->status & kFatal) continue;
-    const int32_t c = m.ctr->cur + 1;
-    m.ctr->cur = c == 4 ? 1 : c;
-    m.ctr->scans_applied += 1;
+    MapCounters BXG_G* ctr = gp(load_view(sd[s].map).ctr);
+    if (ctr->status & kFatal) continue;
+    const int32_t c = ctr->cur + 1;
+    ctr->cur = c == 4 ? 1 : c;
+    ctr->scans_applied += 1;
   }
-  if (threadIdx.x == 0) sc.counters[0] = 0;
+  if (threadIdx.x <= 4 * kRegions) sc.counters[threadIdx.x] = 0;
 }
 
 // ------------------------------------------------------ bulk cell ops
@@ -566,7 +866,7 @@ __global__ void __launch_bounds__(256) k_touch(const MapDev* __restrict__ mp,
                                                const int32_t* __restrict__ coords, uint64_t n) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const int32_t x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
   uint64_t key;
   if (!pack_root(x, y, z, m.log2n, &key)) {
@@ -575,9 +875,8 @@ __global__ void __launch_bounds__(256) k_touch(const MapDev* __restrict__ mp,
   }
   const int32_t slot = root_insert(m, key);
   if (slot < 0) return;
-  const int32_t lmask = ~((1 << m.leaf_bits) - 1);
-  leaf_request(m, (uint32_t)slot, inner_index(x, y, z, m.leaf_bits, m.inner_bits), x & lmask,
-               y & lmask, z & lmask);
+  const int32_t lmask = ~((1 << m.lb) - 1);
+  leaf_request(m, (uint32_t)slot, inner_index(x, y, z, m.lb, m.ib), x & lmask, y & lmask, z & lmask);
 }
 
 // first and last input index of every distinct cell, and its prior mask bit
@@ -586,7 +885,7 @@ __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ m
                                                     uint64_t n, OpScratch os) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const int32_t x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
   const uint32_t leaf = find_leaf(m, x, y, z);
   if (leaf == kNone) {
@@ -594,12 +893,12 @@ __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ m
     os.prebit[i] = 0;
     return;
   }
-  const uint32_t li = leaf_index(x, y, z, m.leaf_bits);
-  const uint64_t key = (uint64_t)leaf * m.leaf_size + li;
-  os.prebit[i] = (uint8_t)((m.leaf_masks[(uint64_t)leaf * m.leaf_words + (li >> 6)] >> (li & 63)) & 1);
+  const uint32_t li = leaf_index(x, y, z, m.lb);
+  const uint64_t key = (uint64_t)leaf * m.lsz + li;
+  os.prebit[i] = (uint8_t)((m.masks[(uint64_t)leaf * m.lw + (li >> 6)] >> (li & 63)) & 1);
   uint32_t h = (uint32_t)mix64(key) & os.mask;
   for (uint32_t p = 0; p <= os.mask; ++p) {
-    uint64_t k = ld64(&os.keys[h]);
+    uint64_t k = gp(os.keys)[h];
     if (k == kEmptyKey) {
       k = cas64(&os.keys[h], kEmptyKey, key);
       if (k == kEmptyKey) k = key;
@@ -617,16 +916,7 @@ __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ m
 
 enum : uint32_t { kOpSetValue = 0, kOpSetOn = 1, kOpSetOff = 2 };
 
-__device__ __forceinline__ void copy_value(uint8_t* dst, const uint8_t* src, uint32_t vb) {
-  if (vb == 4) {
-    *(uint32_t*)dst = *(const uint32_t*)src;
-  } else if (vb == 8) {
-    *(uint64_t*)dst = *(const uint64_t*)src;
-  } else {
-    for (uint32_t b = 0; b < vb; ++b) dst[b] = src[b];
-  }
-}
-__device__ __forceinline__ void copy_value_unaligned(uint8_t* dst, const uint8_t* src, uint32_t vb) {
+__device__ __forceinline__ void copy_bytes(uint8_t BXG_G* dst, const uint8_t BXG_G* src, uint32_t vb) {
   for (uint32_t b = 0; b < vb; ++b) dst[b] = src[b];
 }
 
@@ -641,13 +931,9 @@ __global__ void __launch_bounds__(256) k_op_apply(const MapDev* __restrict__ mp,
                                                   OpScratch os) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const uint32_t t = os.tpos[i];
-  if (t == kNone) {
-    if (was_on) was_on[i] = 0;
-    return;
-  }
-  if (mp->ctr->status & kFatal) {  // the call failed: leave every cell as it was
+  if (t == kNone || (map_status(m) & kFatal)) {  // no cell, or the call failed
     if (was_on) was_on[i] = 0;
     return;
   }
@@ -659,11 +945,11 @@ __global__ void __launch_bounds__(256) k_op_apply(const MapDev* __restrict__ mp,
     if (was_on) was_on[i] = 0;
     return;
   }
-  const uint32_t li = leaf_index(x, y, z, m.leaf_bits);
-  const uint64_t g = (uint64_t)leaf * m.leaf_size + li;
-  uint64_t* mw = &m.leaf_masks[(uint64_t)leaf * m.leaf_words + (li >> 6)];
+  const uint32_t li = leaf_index(x, y, z, m.lb);
+  const uint64_t g = (uint64_t)leaf * m.lsz + li;
+  uint64_t* mw = (uint64_t*)&m.masks[(uint64_t)leaf * m.lw + (li >> 6)];
   const uint64_t bit = 1ull << (li & 63);
-  const uint32_t vb = m.value_bytes;
+  const uint32_t vb = m.vb;
   if (op == kOpSetOff) {
     if (was_on) was_on[i] = (uint8_t)((uint32_t)i == first ? pre : 0);
     if ((uint32_t)i == last) {
@@ -675,12 +961,15 @@ __global__ void __launch_bounds__(256) k_op_apply(const MapDev* __restrict__ mp,
   if (was_on) was_on[i] = (uint8_t)((uint32_t)i == first ? pre : 1);
   if (op == kOpSetValue) {
     if ((uint32_t)i == last) {
-      if (vb) copy_value_unaligned(m.cells + g * vb, vals + i * vb, vb);
+      if (vb == 4)
+        ((uint32_t BXG_G*)m.cells)[g] = gp((const uint32_t*)vals)[i];
+      else if (vb)
+        copy_bytes(m.cells + g * vb, gp(vals) + i * vb, vb);
       or64(mw, bit);
     }
   } else {  // setCellOn: the first occurrence decides, and writes only if off
     if ((uint32_t)i == first) {
-      if (!pre && vb && vals) copy_value_unaligned(m.cells + g * vb, vals + i * vb, vb);
+      if (!pre && vb && vals) copy_bytes(m.cells + g * vb, gp(vals) + i * vb, vb);
       or64(mw, bit);
     }
   }
@@ -706,23 +995,23 @@ __global__ void __launch_bounds__(256) k_lookup(const MapDev* __restrict__ mp,
                                                 uint8_t* __restrict__ found) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const int32_t x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
   const uint32_t leaf = find_leaf(m, x, y, z);
-  const uint32_t vb = m.value_bytes;
+  const uint32_t vb = m.vb;
   bool on = false;
   uint64_t g = 0;
   if (leaf != kNone) {
-    const uint32_t li = leaf_index(x, y, z, m.leaf_bits);
-    g = (uint64_t)leaf * m.leaf_size + li;
-    on = (m.leaf_masks[(uint64_t)leaf * m.leaf_words + (li >> 6)] >> (li & 63)) & 1;
+    const uint32_t li = leaf_index(x, y, z, m.lb);
+    g = (uint64_t)leaf * m.lsz + li;
+    on = (m.masks[(uint64_t)leaf * m.lw + (li >> 6)] >> (li & 63)) & 1;
   }
   if (found) found[i] = on;
   if (vals && vb) {
     if (vb == 4) {
-      ((uint32_t*)vals)[i] = on ? ((const uint32_t*)m.cells)[g] : 0u;
+      ((uint32_t*)vals)[i] = on ? ((const uint32_t BXG_G*)m.cells)[g] : 0u;
     } else if (vb == 8) {
-      ((uint64_t*)vals)[i] = on ? ((const uint64_t*)m.cells)[g] : 0ull;
+      ((uint64_t*)vals)[i] = on ? ((const uint64_t BXG_G*)m.cells)[g] : 0ull;
     } else {
       for (uint32_t b = 0; b < vb; ++b) vals[i * vb + b] = on ? m.cells[g * vb + b] : 0;
     }
@@ -736,15 +1025,15 @@ __global__ void __launch_bounds__(256) k_classify_cells(const MapDev* __restrict
                                                         uint64_t n, int8_t* __restrict__ state) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const int32_t x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
   const uint32_t leaf = find_leaf(m, x, y, z);
   int8_t st = 0;
   if (leaf != kNone) {
-    const uint32_t li = leaf_index(x, y, z, m.leaf_bits);
-    if ((m.leaf_masks[(uint64_t)leaf * m.leaf_words + (li >> 6)] >> (li & 63)) & 1) {
-      const int32_t p = cell_p(((const uint32_t*)m.cells)[(uint64_t)leaf * m.leaf_size + li]);
-      st = p > m.opt_thr ? 1 : (p < m.opt_thr ? -1 : 0);
+    const uint32_t li = leaf_index(x, y, z, m.lb);
+    if ((m.masks[(uint64_t)leaf * m.lw + (li >> 6)] >> (li & 63)) & 1) {
+      const int32_t p = cell_p(((const uint32_t BXG_G*)m.cells)[(uint64_t)leaf * m.lsz + li]);
+      st = p > m.thr ? 1 : (p < m.thr ? -1 : 0);
     }
   }
   state[i] = st;
@@ -758,44 +1047,44 @@ __global__ void __launch_bounds__(256) k_export(const MapDev* __restrict__ mp, u
                                                 uint32_t filter, int32_t* __restrict__ coords,
                                                 uint8_t* __restrict__ vals, uint64_t cap,
                                                 uint32_t* __restrict__ count) {
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t leaf = t >> (3 * m.leaf_bits);
-  const uint32_t li = (uint32_t)(t & (m.leaf_size - 1));
+  const uint64_t leaf = t >> (3 * m.lb);
+  const uint32_t li = (uint32_t)(t & (m.lsz - 1));
   bool pass = false;
   if (leaf < nleaves) {
-    pass = (m.leaf_masks[leaf * m.leaf_words + (li >> 6)] >> (li & 63)) & 1;
+    pass = (m.masks[leaf * m.lw + (li >> 6)] >> (li & 63)) & 1;
     if (pass && filter) {
-      const int32_t p = cell_p(((const uint32_t*)m.cells)[leaf * m.leaf_size + li]);
-      pass = filter == 1 ? p > m.opt_thr : p < m.opt_thr;
+      const int32_t p = cell_p(((const uint32_t BXG_G*)m.cells)[leaf * m.lsz + li]);
+      pass = filter == 1 ? p > m.thr : p < m.thr;
     }
   }
   const uint32_t pos = wave_append(count, pass);
   if (!pass || pos >= cap) return;
-  const int4 o = m.leaf_origin[leaf];
-  const uint32_t lb = m.leaf_bits, lm = (1u << lb) - 1;
+  const int4 o = m.origin[leaf];
+  const uint32_t lb = m.lb, lm = (1u << lb) - 1;
   if (coords) {
     coords[3 * (uint64_t)pos] = o.x | (int32_t)(li & lm);
     coords[3 * (uint64_t)pos + 1] = o.y | (int32_t)((li >> lb) & lm);
     coords[3 * (uint64_t)pos + 2] = o.z | (int32_t)((li >> (2 * lb)) & lm);
   }
-  const uint32_t vb = m.value_bytes;
-  if (vals && vb) copy_value_unaligned(vals + (uint64_t)pos * vb, m.cells + (leaf * m.leaf_size + li) * vb, vb);
+  const uint32_t vb = m.vb;
+  if (vals && vb) copy_bytes(gp(vals) + (uint64_t)pos * vb, m.cells + (leaf * m.lsz + li) * vb, vb);
 }
 
 // order-free digest of the on cells (count, sum, xor, sum of remix)
 __global__ void __launch_bounds__(256) k_digest(const MapDev* __restrict__ mp, uint32_t nleaves,
                                                 uint64_t* __restrict__ out) {
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t leaf = t >> (3 * m.leaf_bits);
-  const uint32_t li = (uint32_t)(t & (m.leaf_size - 1));
+  const uint64_t leaf = t >> (3 * m.lb);
+  const uint32_t li = (uint32_t)(t & (m.lsz - 1));
   uint64_t h = 0, h2 = 0, c = 0;
-  if (leaf < nleaves && ((m.leaf_masks[leaf * m.leaf_words + (li >> 6)] >> (li & 63)) & 1)) {
-    const int4 o = m.leaf_origin[leaf];
-    const uint32_t lb = m.leaf_bits, lm = (1u << lb) - 1, vb = m.value_bytes;
+  if (leaf < nleaves && ((m.masks[leaf * m.lw + (li >> 6)] >> (li & 63)) & 1)) {
+    const int4 o = m.origin[leaf];
+    const uint32_t lb = m.lb, lm = (1u << lb) - 1, vb = m.vb;
     uint64_t word = 0;
-    const uint8_t* src = m.cells + (leaf * m.leaf_size + li) * vb;
+    const uint8_t BXG_G* src = m.cells + (leaf * m.lsz + li) * vb;
     if (vb <= 8) {
       for (uint32_t b = 0; b < vb; ++b) word |= (uint64_t)src[b] << (8 * b);
     } else {
@@ -829,12 +1118,12 @@ __global__ void __launch_bounds__(256) k_digest(const MapDev* __restrict__ mp, u
 // activeCellsCount (bonxai.hpp:689-701): popcount of every leaf mask word
 __global__ void __launch_bounds__(256) k_count_on(const MapDev* __restrict__ mp, uint32_t nleaves,
                                                   uint64_t* __restrict__ out) {
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   uint64_t c = 0;
-  if (t < (uint64_t)nleaves * m.leaf_words) {
-    const uint64_t w = m.leaf_masks[t];
-    c = m.leaf_size < 64 ? (uint64_t)__popcll(w & ((1ull << m.leaf_size) - 1)) : (uint64_t)__popcll(w);
+  if (t < (uint64_t)nleaves * m.lw) {
+    const uint64_t w = m.masks[t];
+    c = m.lsz < 64 ? (uint64_t)__popcll(w & ((1ull << m.lsz) - 1)) : (uint64_t)__popcll(w);
   }
   for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
   if (lane_id() == 0 && c) add64(out, c);
@@ -842,12 +1131,12 @@ __global__ void __launch_bounds__(256) k_count_on(const MapDev* __restrict__ mp,
 
 // clear(SET_ALL_CELLS_OFF): every mask word and every cell byte to zero
 __global__ void __launch_bounds__(256) k_cells_off(const MapDev* __restrict__ mp, uint32_t nleaves) {
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t nw = (uint64_t)nleaves * m.leaf_words;
-  for (uint64_t w = t; w < nw; w += (uint64_t)gridDim.x * blockDim.x) m.leaf_masks[w] = 0;
-  const uint64_t nb = (uint64_t)nleaves * m.leaf_size * m.value_bytes / 4;
-  for (uint64_t b = t; b < nb; b += (uint64_t)gridDim.x * blockDim.x) ((uint32_t*)m.cells)[b] = 0;
+  const uint64_t nw = (uint64_t)nleaves * m.lw;
+  for (uint64_t w = t; w < nw; w += (uint64_t)gridDim.x * blockDim.x) m.masks[w] = 0;
+  const uint64_t nb = (uint64_t)nleaves * m.lsz * m.vb / 4;
+  for (uint64_t b = t; b < nb; b += (uint64_t)gridDim.x * blockDim.x) ((uint32_t BXG_G*)m.cells)[b] = 0;
 }
 
 // re-insert every root of an old directory into the (empty) directory of m
@@ -859,13 +1148,13 @@ __global__ void __launch_bounds__(256) k_rehash(const MapDev* __restrict__ mp,
   if (s >= old_cap) return;
   const uint64_t key = old_keys[s];
   if (key == kEmptyKey) return;
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const int32_t ns = root_insert(m, key);
   if (ns < 0) return;
-  for (uint32_t c = 0; c < m.inner_size; ++c) {
-    const uint32_t leaf = old_children[(uint64_t)s * m.inner_size + c];
-    m.children[(uint64_t)ns * m.inner_size + c] = leaf;
-    if (leaf < kPendingChild) m.leaf_origin[leaf].w = ns;
+  for (uint32_t c = 0; c < m.isz; ++c) {
+    const uint32_t leaf = old_children[(uint64_t)s * m.isz + c];
+    m.children[(uint64_t)ns * m.isz + c] = leaf;
+    if (leaf < kPendingChild) m.origin[leaf].w = ns;
   }
 }
 
@@ -880,6 +1169,14 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
     p[i] = v;
 }
 
+// a plain 16-byte-per-lane copy: calibration of the HBM byte counters
+__global__ void __launch_bounds__(256) k_copy_probe(const uint4* __restrict__ src,
+                                                    uint4* __restrict__ dst, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 // ----------------------------------------------------- leaf page I/O
 
 // Copy whole leaves (mask words + cell bytes) named by their origin coords
@@ -889,14 +1186,14 @@ __global__ void __launch_bounds__(256) k_read_leaves(const MapDev* __restrict__ 
                                                      uint64_t* __restrict__ masks,
                                                      uint8_t* __restrict__ cells,
                                                      uint8_t* __restrict__ exists) {
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const uint32_t k = blockIdx.x;
   if (k >= n) return;
   const uint32_t leaf = find_leaf(m, origins[3 * k], origins[3 * k + 1], origins[3 * k + 2]);
   if (threadIdx.x == 0) exists[k] = leaf != kNone;
-  for (uint32_t w = threadIdx.x; w < m.leaf_words; w += blockDim.x)
-    masks[(uint64_t)k * m.leaf_words + w] = leaf != kNone ? m.leaf_masks[(uint64_t)leaf * m.leaf_words + w] : 0;
-  const uint64_t nb = (uint64_t)m.leaf_size * m.value_bytes;
+  for (uint32_t w = threadIdx.x; w < m.lw; w += blockDim.x)
+    masks[(uint64_t)k * m.lw + w] = leaf != kNone ? m.masks[(uint64_t)leaf * m.lw + w] : 0;
+  const uint64_t nb = (uint64_t)m.lsz * m.vb;
   for (uint64_t b = threadIdx.x; b < nb; b += blockDim.x)
     cells[k * nb + b] = leaf != kNone ? m.cells[leaf * nb + b] : 0;
 }
@@ -907,18 +1204,18 @@ __global__ void __launch_bounds__(256) k_write_leaves(const MapDev* __restrict__
                                                       const int32_t* __restrict__ origins,
                                                       uint32_t n, const uint64_t* __restrict__ masks,
                                                       const uint8_t* __restrict__ cells) {
-  const MapDev& m = *mp;
+  const MapView m = load_view(mp);
   const uint32_t k = blockIdx.x;
   if (k >= n) return;
   const uint32_t leaf = find_leaf(m, origins[3 * k], origins[3 * k + 1], origins[3 * k + 2]);
   if (leaf == kNone) return;
-  for (uint32_t w = threadIdx.x; w < m.leaf_words; w += blockDim.x)
-    m.leaf_masks[(uint64_t)leaf * m.leaf_words + w] = masks[(uint64_t)k * m.leaf_words + w];
-  const uint32_t vb = m.value_bytes;
-  const uint64_t nb = (uint64_t)m.leaf_size * vb;
+  for (uint32_t w = threadIdx.x; w < m.lw; w += blockDim.x)
+    m.masks[(uint64_t)leaf * m.lw + w] = masks[(uint64_t)k * m.lw + w];
+  const uint32_t vb = m.vb;
+  const uint64_t nb = (uint64_t)m.lsz * vb;
   for (uint64_t b = threadIdx.x; b < nb; b += blockDim.x) {
     const uint32_t li = (uint32_t)(b / vb);
-    const bool on = (masks[(uint64_t)k * m.leaf_words + (li >> 6)] >> (li & 63)) & 1;
+    const bool on = (masks[(uint64_t)k * m.lw + (li >> 6)] >> (li & 63)) & 1;
     m.cells[leaf * nb + b] = on ? cells[k * nb + b] : 0;
   }
 }
@@ -928,7 +1225,8 @@ __global__ void __launch_bounds__(256) k_leaf_origins(const MapDev* __restrict__
                                                       int32_t* __restrict__ out) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nleaves) return;
-  const int4 o = mp->leaf_origin[k];
+  const MapView m = load_view(mp);
+  const int4 o = m.origin[k];
   out[3 * k] = o.x;
   out[3 * k + 1] = o.y;
   out[3 * k + 2] = o.z;

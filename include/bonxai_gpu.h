@@ -125,6 +125,11 @@ int bxg_profile_read(bxg_ctx* ctx, const char* kernel, uint64_t* launches, doubl
 int bxg_profile_reset(bxg_ctx* ctx);
 /* exact per-scan counting of ray updates (slower; for statistics runs) */
 int bxg_set_counting(bxg_ctx* ctx, int on);
+/* launch tuning of the context, set by the autotuner: "ray_blocks_per_cu"
+ * and "leaf_blocks_per_cu" (1..256), the workgroups per CU of the
+ * work-stealing update pass and leaf-allocation pass over the rays */
+int bxg_ctx_set_tuning(bxg_ctx* ctx, const char* key, int64_t value);
+int bxg_ctx_get_tuning(bxg_ctx* ctx, const char* key, int64_t* value);
 
 /* -------------------------------------------------------------- maps */
 
@@ -240,6 +245,10 @@ int bxg_graph_launch(bxg_graph* g);
 int bxg_graph_destroy(bxg_graph* g);
 
 /* ---------------------------------------------------------- checks */
+
+/* a plain device-to-device copy kernel (16 bytes a lane), bytes % 16 == 0:
+ * the known-size reference that scales the HBM byte counters of a profile */
+int bxg_copy_probe(bxg_ctx* ctx, const void* src_dev, void* dst_dev, uint64_t bytes);
 
 /* element-wise FP64 operations of the scan front end on device arrays:
  * out[5i..5i+4] = sqrt(a), a/b, c + (a/b)*c, floor(a*b), (a*a + b*b) + c*c */
