@@ -28,6 +28,18 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE and
+    WRITE_SIZE passes (profiles/traffic.json, scaled by the copy-probe factors), or
+    None when no profile has been recorded."""
+    try:
+        with open(os.path.join(HERE, "profiles", "traffic.json")) as f:
+            return json.load(f)[kernel]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
 METRIC = "rays/s + voxel updates/s, batched vs plain per-scan on 1 MI355X; p50/p99 latency"
 
 
@@ -83,8 +95,9 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range):
         bx.insert_batch(ctx, descs)
     ctx.profile(False)
     kern = {}
-    for name in ("k_scan_classify", "k_scan_winners", "k_scan_ray_leaves", "k_scan_endpoints",
-                 "k_scan_rays", "k_scan_finalize"):
+    for name in ("k_scan_classify", "k_scan_winners", "k_scan_bucket_reduce", "k_scan_bucket_scan",
+                 "k_scan_scatter", "k_scan_ray_leaves", "k_scan_endpoints", "k_scan_rays",
+                 "k_scan_finalize"):
         n, ms = ctx.profile_read(name)
         kern[name] = ms / max(n, 1)
     g.close()
@@ -275,7 +288,8 @@ def main():
         "kernels_ms": b["kernels_ms"],
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_scan_rays",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_scan_rays"),
+            "kernel": "k_scan_rays",
             "alg_bytes_per_launch": alg_bytes,
         },
         "cpu_baseline": {
@@ -295,6 +309,7 @@ def main():
         lk_gbs = lk_bytes / (lk["lookup_ms"] / 1e3) / 1e9
         out["lookup"] = {**lk, "roofline": {"bound": "hbm", "achieved": lk_gbs, "peak": HBM_PEAK_GBS,
                                             "unit": "GB/s", "frac": lk_gbs / HBM_PEAK_GBS,
+                                            "traffic": measured_traffic("k_lookup"),
                                             "bytes_per_lookup": 28}}
     for d in dev:
         d.free()

@@ -23,9 +23,6 @@ using namespace bxg;
 namespace {
 
 constexpr uint32_t kBlock = 256;
-// scan counters: winner count + two sets of per-region cursors
-constexpr uint32_t kCounterWords = 64;
-static_assert(1 + 4 * kRegions <= kCounterWords, "counter words");
 constexpr int kStageSlots = 8;
 
 struct PinnedSlot {
@@ -625,7 +622,8 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
   const uint32_t gl = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->leaf_blocks_per_cu, most));
   BXG_TRY(launch(c, "k_scan_classify", k_scan_classify, gp, kBlock, hdr, desc, sc));
   BXG_TRY(launch(c, "k_scan_winners", k_scan_winners, gp, kBlock, hdr, desc, sc));
-  BXG_TRY(launch(c, "k_scan_bucket_scan", k_scan_bucket_scan, 1, 1024, hdr, sc));
+  BXG_TRY(launch(c, "k_scan_bucket_reduce", k_scan_bucket_reduce, kTiles, 1024, hdr, sc));
+  BXG_TRY(launch(c, "k_scan_bucket_scan", k_scan_bucket_scan, kTiles, 1024, hdr, sc));
   BXG_TRY(launch(c, "k_scan_scatter", k_scan_scatter, gp, kBlock, hdr, sc));
   BXG_TRY(launch(c, "k_scan_ray_leaves", k_scan_ray_leaves, gl, kBlock, desc, sc));
   if (any_extra)

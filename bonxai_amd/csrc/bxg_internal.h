@@ -112,6 +112,14 @@ constexpr uint32_t kDirBits = 12;  // 64 x 64 octahedral direction bins
 constexpr uint32_t kMaxTagBits = 6;
 constexpr uint32_t kMaxBuckets = 1u << (kDirBits + kMaxTagBits);
 constexpr uint32_t kRegions = 8;   // winner-list regions, one per XCD
+constexpr uint32_t kScanTile = 4096;  // buckets one block of the bucket scan covers
+constexpr uint32_t kTiles = kMaxBuckets / kScanTile;
+static_assert((1u << kDirBits) % kScanTile == 0 && kTiles <= 64, "bucket scan tiles");
+// scan counter words: [0] winner count, [1, 1 + 4 * kRegions) region cursors
+// and exhausted flags of the two walks, [kTileWords, + kTiles) bucket-tile sums
+constexpr uint32_t kTileWords = 64;
+constexpr uint32_t kCounterWords = kTileWords + kTiles;
+static_assert(1 + 4 * kRegions <= kTileWords, "counter words");
 
 // scratch of one integration call, owned by the context
 struct ScanScratch {

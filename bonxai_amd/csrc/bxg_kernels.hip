@@ -567,34 +567,75 @@ __global__ void __launch_bounds__(256) k_scan_winners(const BatchHdr* __restrict
   wave_add_by_key(sc.hist, key, win);
 }
 
-// Phase 2s (one block): exclusive scan of the bucket counts into write
-// cursors; the winner count goes to counters[0] and the counts are zeroed.
+// inclusive prefix sum over the 64 lanes of a wavefront
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v) {
+  const uint32_t lane = threadIdx.x & 63u;
+#pragma unroll
+  for (uint32_t o = 1; o < 64u; o <<= 1) {
+    const uint32_t t = (uint32_t)__shfl_up((int)v, o);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
+// Phase 2s (per bucket tile): sum of the winners of each tile of kScanTile
+// buckets, four buckets a thread.
+__global__ void __launch_bounds__(1024) k_scan_bucket_reduce(const BatchHdr* __restrict__ hdr,
+                                                             ScanScratch sc) {
+  __shared__ uint32_t wsum[16];
+  const uint32_t nb = 1u << (kDirBits + hdr->tag_bits);
+  const uint32_t base = blockIdx.x * kScanTile;
+  if (base >= nb) return;
+  const uint4 v = reinterpret_cast<const uint4*>(sc.hist + base)[threadIdx.x];
+  const uint32_t s = wave_scan(v.x + v.y + v.z + v.w);
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  if (lane == 63) wsum[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < 16; ++w) t += wsum[w];
+    sc.counters[kTileWords + blockIdx.x] = t;
+  }
+}
+
+// Phase 2s' (per bucket tile): exclusive scan of the bucket counts into the
+// bucket write cursors, carried over the earlier tiles; the counts are zeroed
+// for the next call and the last tile writes the winner count to counters[0].
 __global__ void __launch_bounds__(1024) k_scan_bucket_scan(const BatchHdr* __restrict__ hdr,
                                                            ScanScratch sc) {
-  __shared__ uint32_t part[1024];
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t carry;
   const uint32_t nb = 1u << (kDirBits + hdr->tag_bits);
-  const uint32_t per = (nb + 1023u) / 1024u;
-  const uint32_t t = threadIdx.x;
-  const uint32_t lo = t * per, hi = lo + per < nb ? lo + per : nb;
-  uint32_t sum = 0;
-  for (uint32_t i = lo; i < hi; ++i) sum += sc.hist[i];
-  part[t] = sum;
+  const uint32_t base = blockIdx.x * kScanTile;
+  if (base >= nb) return;
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  if (wid == 0) {
+    // blockIdx.x < kTiles <= 64: one lane per earlier tile
+    const uint32_t t = lane < blockIdx.x ? sc.counters[kTileWords + lane] : 0u;
+    const uint32_t tot = (uint32_t)__shfl((int)wave_scan(t), 63);
+    if (lane == 0) carry = tot;
+  }
+  uint4* h4 = reinterpret_cast<uint4*>(sc.hist + base);
+  const uint4 v = h4[threadIdx.x];
+  const uint32_t s = v.x + v.y + v.z + v.w;
+  const uint32_t incl = wave_scan(s);
+  if (lane == 63) wsum[wid] = incl;
   __syncthreads();
-  // Hillis-Steele inclusive scan of the 1024 partial sums
-  for (uint32_t off = 1; off < 1024u; off <<= 1) {
-    const uint32_t v = t >= off ? part[t - off] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  if (wid == 0) {
+    const uint32_t w = lane < 16u ? wsum[lane] : 0u;
+    const uint32_t wi = wave_scan(w);
+    if (lane < 16u) wsum[lane] = wi;
   }
-  uint32_t run = part[t] - sum;
-  for (uint32_t i = lo; i < hi; ++i) {
-    const uint32_t c = sc.hist[i];
-    sc.offs[i] = run;
-    sc.hist[i] = 0;
-    run += c;
-  }
-  if (t == 1023) sc.counters[0] = part[1023];
+  __syncthreads();
+  uint32_t run = carry + (wid ? wsum[wid - 1] : 0u) + incl - s;
+  uint4 o;
+  o.x = run; run += v.x;
+  o.y = run; run += v.y;
+  o.z = run; run += v.z;
+  o.w = run;
+  reinterpret_cast<uint4*>(sc.offs + base)[threadIdx.x] = o;
+  h4[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
+  if (base + kScanTile >= nb && threadIdx.x == 1023) sc.counters[0] = carry + wsum[15];
 }
 
 // Phase 2t (per point): winners written into their buckets.
