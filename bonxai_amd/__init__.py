@@ -447,6 +447,11 @@ class VoxelGrid:
     def clear(self, option=CLEAR_MEMORY):
         _check(lib().bxg_map_clear(self._h, option), self.ctx, "clear")
 
+    def release_unused_memory(self):
+        """releaseUnusedMemory (bonxai.hpp:367-387): leaves with every cell off
+        and roots left without leaves are freed; the pools shrink."""
+        _check(lib().bxg_map_release_unused(self._h), self.ctx, "release_unused_memory")
+
     def reserve(self, leaves, roots=0):
         _check(lib().bxg_map_reserve(self._h, leaves, roots), self.ctx, "reserve")
 

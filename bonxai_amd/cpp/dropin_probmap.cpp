@@ -1,0 +1,177 @@
+// Scan integration through the drop-in C++ API: Bonxai::ProbabilisticMap driven
+// the way a reference caller drives it (insertPointCloud with float or double
+// point types, addHitPoint / addMissPoint, isOccupied / isFree / isUnknown,
+// getOccupiedVoxels / getFreeVoxels, accessors on grid()), replaying a script
+// and dumping the results for comparison with an independent model.
+//
+// usage: dropin_probmap script.bin result.bin
+//
+// script (little-endian):  double resolution, uint32 op count, then per op a
+// uint32 kind and its payload:
+//   0 insert, float points   uint32 n, double origin[3], double max_range, float xyz[3n]
+//   1 insert, double points  uint32 n, double origin[3], double max_range, double xyz[3n]
+//   2 addHitPoint            double xyz[3]
+//   3 addMissPoint           double xyz[3]
+//   4 query                  uint32 n, int32 xyz[3n]
+//   5 accessor write         uint32 n, int32 xyz[3n], int32 word[n]  (value(coord, true) = word)
+// result: per query op, per coordinate: int8 isOccupied, isFree, isUnknown,
+// classify state, int64 cell word through a const accessor (-1 if off); then
+// uint64 n + n x (int32 xyz, int32 word) from forEachCell, uint64 n + n x int32
+// xyz from getOccupiedVoxels, the same from getFreeVoxels, uint64 n + n x double
+// xyz from getOccupiedVoxels<Point3D>, uint64 activeCellsCount, uint32 update count.
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+#include <vector>
+
+#include "bonxai_map/probabilistic_map.hpp"
+
+namespace {
+
+struct FloatPoint {  // a pcl::PointXYZ-like type with float members
+  float x, y, z;
+};
+
+struct Reader {
+  std::ifstream in;
+  explicit Reader(const char* path) : in(path, std::ios::binary) {
+    if (!in) throw std::runtime_error("cannot open script");
+  }
+  template <typename T>
+  T get() {
+    T v{};
+    in.read(reinterpret_cast<char*>(&v), sizeof(T));
+    if (!in) throw std::runtime_error("truncated script");
+    return v;
+  }
+  template <typename T>
+  void get(T* p, size_t n) {
+    in.read(reinterpret_cast<char*>(p), sizeof(T) * n);
+    if (!in) throw std::runtime_error("truncated script");
+  }
+};
+
+struct Writer {
+  std::ofstream out;
+  explicit Writer(const char* path) : out(path, std::ios::binary) {}
+  template <typename T>
+  void put(const T& v) {
+    out.write(reinterpret_cast<const char*>(&v), sizeof(T));
+  }
+};
+
+int32_t word_of(const Bonxai::ProbabilisticMap::CellT& c) {
+  int32_t w;
+  std::memcpy(&w, &c, 4);
+  return w;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    std::cerr << "usage: dropin_probmap script.bin result.bin" << std::endl;
+    return 2;
+  }
+  try {
+    Reader r(argv[1]);
+    Writer w(argv[2]);
+    const double res = r.get<double>();
+    Bonxai::ProbabilisticMap map(res);
+    using CellT = Bonxai::ProbabilisticMap::CellT;
+    const uint32_t nops = r.get<uint32_t>();
+    for (uint32_t op = 0; op < nops; ++op) {
+      const uint32_t kind = r.get<uint32_t>();
+      if (kind == 0 || kind == 1) {
+        const uint32_t n = r.get<uint32_t>();
+        double o[3];
+        r.get(o, 3);
+        const double range = r.get<double>();
+        if (kind == 0) {
+          std::vector<FloatPoint> pts(n);
+          r.get(reinterpret_cast<float*>(pts.data()), 3 * size_t(n));
+          const FloatPoint origin{float(o[0]), float(o[1]), float(o[2])};
+          if (double(origin.x) != o[0] || double(origin.y) != o[1] || double(origin.z) != o[2])
+            throw std::runtime_error("float-point scans need a float origin");
+          map.insertPointCloud(pts, origin, range);
+        } else {
+          std::vector<Bonxai::Point3D> pts(n);
+          for (auto& p : pts) {
+            p.x = r.get<double>();
+            p.y = r.get<double>();
+            p.z = r.get<double>();
+          }
+          map.insertPointCloud(pts, Bonxai::Point3D(o[0], o[1], o[2]), range);
+        }
+      } else if (kind == 2 || kind == 3) {
+        double p[3];
+        r.get(p, 3);
+        const Bonxai::ProbabilisticMap::Vector3D v(p[0], p[1], p[2]);
+        kind == 2 ? map.addHitPoint(v) : map.addMissPoint(v);
+      } else if (kind == 4) {
+        const uint32_t n = r.get<uint32_t>();
+        std::vector<Bonxai::CoordT> q(n);
+        r.get(reinterpret_cast<int32_t*>(q.data()), 3 * size_t(n));
+        std::vector<int8_t> state;
+        map.classify(q, state);
+        const auto& grid = map.grid();
+        auto acc = grid.createConstAccessor();
+        for (uint32_t i = 0; i < n; ++i) {
+          w.put(int8_t(map.isOccupied(q[i])));
+          w.put(int8_t(map.isFree(q[i])));
+          w.put(int8_t(map.isUnknown(q[i])));
+          w.put(state[i]);
+          const CellT* c = acc.value(q[i]);
+          w.put(int64_t(c ? word_of(*c) : -1));
+        }
+      } else if (kind == 5) {
+        const uint32_t n = r.get<uint32_t>();
+        std::vector<Bonxai::CoordT> q(n);
+        r.get(reinterpret_cast<int32_t*>(q.data()), 3 * size_t(n));
+        std::vector<int32_t> words(n);
+        r.get(words.data(), n);
+        auto acc = map.grid().createAccessor();
+        for (uint32_t i = 0; i < n; ++i) {
+          CellT* c = acc.value(q[i], true);
+          c->update_id = int32_t(uint32_t(words[i]) << 28) >> 28;
+          c->probability_log = words[i] >> 4;
+        }
+      } else {
+        throw std::runtime_error("unknown op");
+      }
+    }
+    std::vector<std::pair<Bonxai::CoordT, int32_t>> cells;
+    map.grid().forEachCell([&](const CellT& c, const Bonxai::CoordT& xyz) { cells.push_back({xyz, word_of(c)}); });
+    w.put(uint64_t(cells.size()));
+    for (const auto& c : cells) {
+      w.put(c.first);
+      w.put(c.second);
+    }
+    std::vector<Bonxai::CoordT> occ, fre;
+    map.getOccupiedVoxels(occ);
+    map.getFreeVoxels(fre);
+    for (auto* v : {&occ, &fre}) {
+      w.put(uint64_t(v->size()));
+      for (const auto& c : *v) w.put(c);
+    }
+    std::vector<Bonxai::Point3D> pts;
+    map.getOccupiedVoxels(pts);
+    w.put(uint64_t(pts.size()));
+    for (const auto& p : pts) {
+      w.put(p.x);
+      w.put(p.y);
+      w.put(p.z);
+    }
+    w.put(uint64_t(map.grid().activeCellsCount()));
+    w.put(map.updateCount());
+    std::cout << "ops " << nops << " cells " << cells.size() << " occupied " << occ.size() << " free "
+              << fre.size() << std::endl;
+  } catch (const std::exception& e) {
+    std::cerr << "error: " << e.what() << std::endl;
+    return 1;
+  }
+  return 0;
+}

@@ -400,7 +400,7 @@ int grow_roots(bxg_map* m, uint64_t need_roots) {
   uint32_t zero = 0;
   BXG_HIP(c, hipMemcpyAsync(&m->ctr->root_count, &zero, 4, hipMemcpyHostToDevice, c->stream));
   if (old_keys) BXG_TRY(launch(c, "k_rehash", k_rehash, nblocks(old_cap), kBlock, (const MapDev*)m->d,
-                               (const uint64_t*)old_keys, (const uint32_t*)old_children, old_cap));
+                               (const uint64_t*)old_keys, (const uint32_t*)old_children, old_cap, 0u));
   BXG_TRY(ctx_sync(c));
   dfree(old_keys);
   dfree(old_children);
@@ -1224,7 +1224,84 @@ int bxg_map_digest(bxg_map* m, uint64_t out[4]) {
 
 int bxg_map_release_unused(bxg_map* m) {
   if (!m) return BXG_ERR_INVALID_ARG;
-  return fail(m->ctx, BXG_ERR_NOT_SUPPORTED, "releaseUnusedMemory is not implemented yet");
+  bxg_ctx* c = m->ctx;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(map_failed(m));
+  uint32_t nl = 0;
+  BXG_TRY(known_leaf_count(m, &nl));
+  // 1. live leaves and their order-preserving new indices
+  const uint32_t ntiles = (uint32_t)((nl + kScanTile - 1) / kScanTile);
+  void *dlive = nullptr, *didx = nullptr, *dtiles = nullptr;
+  BXG_TRY(dstage(c, 0, (uint64_t)nl * 4 + 64, &dlive));
+  BXG_TRY(dstage(c, 1, (uint64_t)nl * 4 + 64, &didx));
+  BXG_TRY(dstage(c, 2, (uint64_t)ntiles * 4 + 64, &dtiles));
+  uint32_t live_leaves = 0;
+  if (nl) {
+    BXG_TRY(launch(c, "k_leaf_live", k_leaf_live, nblocks(nl), kBlock, (const MapDev*)m->d, nl,
+                   (uint32_t*)dlive));
+    BXG_TRY(launch(c, "k_tile_reduce", k_tile_reduce, ntiles, 1024, (const uint32_t*)dlive,
+                   (uint64_t)nl, (uint32_t*)dtiles));
+    BXG_TRY(launch(c, "k_tile_sums_scan", k_tile_sums_scan, 1, 1024, (uint32_t*)dtiles, ntiles,
+                   (uint32_t*)c->d_small));
+    BXG_TRY(launch(c, "k_tile_scan", k_tile_scan, ntiles, 1024, (const uint32_t*)dlive, (uint64_t)nl,
+                   (const uint32_t*)dtiles, (uint32_t*)didx));
+    BXG_TRY(d2h_sync(c, c->h_small, c->d_small, 4));
+    std::memcpy(&live_leaves, c->h_small, 4);
+  }
+  // 2. a right-sized leaf pool holding the live leaves at the front
+  const uint64_t new_cap = ((uint64_t)live_leaves * 5 / 4 + 1024 + 255) & ~255ull;
+  const uint64_t lw = m->h.leaf_words, cb = (uint64_t)m->h.leaf_size * m->h.value_bytes;
+  void *masks = nullptr, *cells = nullptr, *orig = nullptr;
+  BXG_TRY(dmalloc(c, &masks, new_cap * lw * 8));
+  if (cb) BXG_TRY(dmalloc(c, &cells, new_cap * cb));
+  BXG_TRY(dmalloc(c, &orig, new_cap * sizeof(int4)));
+  BXG_HIP(c, hipMemsetAsync(masks, 0, new_cap * lw * 8, c->stream));
+  if (cb) BXG_HIP(c, hipMemsetAsync(cells, 0, new_cap * cb, c->stream));
+  if (nl)
+    BXG_TRY(launch(c, "k_compact_leaves", k_compact_leaves, nl, kBlock, (const MapDev*)m->d, nl,
+                   (const uint32_t*)dlive, (const uint32_t*)didx, (uint64_t*)masks,
+                   (uint8_t*)cells, (int4*)orig));
+  // 3. a directory without the roots left empty, sized for what remains
+  BXG_TRY(ctx_sync(c));
+  uint64_t live_roots = m->known_roots;
+  const uint64_t rcap = pow2_at_least(std::max<uint64_t>(live_roots * 2, 1024));
+  void *keys = nullptr, *children = nullptr;
+  BXG_TRY(dmalloc(c, &keys, rcap * 8));
+  BXG_TRY(dmalloc(c, &children, rcap * m->h.inner_size * 4));
+  BXG_TRY(fill64(c, (uint64_t*)keys, rcap, kEmptyKey));
+  BXG_TRY(fill32(c, (uint32_t*)children, rcap * m->h.inner_size, kEmptyChild));
+  uint64_t* old_keys = m->h.root_keys;
+  uint32_t* old_children = m->h.children;
+  const uint32_t old_rcap = m->root_cap;
+  uint64_t* old_masks = m->h.leaf_masks;
+  uint8_t* old_cells = m->h.cells;
+  int4* old_orig = m->h.leaf_origin;
+  const uint64_t old_bytes = (uint64_t)m->leaf_cap * leaf_bytes(m) + (uint64_t)old_rcap * root_bytes(m);
+  m->h.leaf_masks = (uint64_t*)masks;
+  m->h.cells = (uint8_t*)cells;
+  m->h.leaf_origin = (int4*)orig;
+  m->h.leaf_cap = (uint32_t)new_cap;
+  m->leaf_cap = (uint32_t)new_cap;
+  m->h.root_keys = (uint64_t*)keys;
+  m->h.children = (uint32_t*)children;
+  m->h.root_mask = (uint32_t)(rcap - 1);
+  m->h.max_probe = (uint32_t)std::min<uint64_t>(rcap, 512);
+  m->root_cap = (uint32_t)rcap;
+  BXG_TRY(push_header(m));
+  const uint32_t z2[2] = {live_leaves, 0};  // leaf_count, root_count
+  BXG_HIP(c, hipMemcpyAsync(&m->ctr->leaf_count, z2, 8, hipMemcpyHostToDevice, c->stream));
+  BXG_TRY(launch(c, "k_rehash", k_rehash, nblocks(old_rcap), kBlock, (const MapDev*)m->d,
+                 (const uint64_t*)old_keys, (const uint32_t*)old_children, old_rcap, 1u));
+  BXG_TRY(ctx_sync(c));
+  dfree(old_keys);
+  dfree(old_children);
+  dfree(old_masks);
+  dfree(old_cells);
+  dfree(old_orig);
+  const uint64_t new_bytes = new_cap * leaf_bytes(m) + rcap * root_bytes(m);
+  c->used = c->used - old_bytes + new_bytes;
+  m->bytes = m->bytes - old_bytes + new_bytes;
+  return read_counters(m);
 }
 
 // ------------------------------------------------------ bulk VoxelGrid

@@ -1180,22 +1180,148 @@ __global__ void __launch_bounds__(256) k_cells_off(const MapDev* __restrict__ mp
   for (uint64_t b = t; b < nb; b += (uint64_t)gridDim.x * blockDim.x) ((uint32_t BXG_G*)m.cells)[b] = 0;
 }
 
-// re-insert every root of an old directory into the (empty) directory of m
+// re-insert every root of an old directory into the (empty) directory of m;
+// with prune set, roots without any leaf are dropped
 __global__ void __launch_bounds__(256) k_rehash(const MapDev* __restrict__ mp,
                                                 const uint64_t* __restrict__ old_keys,
                                                 const uint32_t* __restrict__ old_children,
-                                                uint32_t old_cap) {
+                                                uint32_t old_cap, uint32_t prune) {
   const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= old_cap) return;
   const uint64_t key = old_keys[s];
   if (key == kEmptyKey) return;
   const MapView m = load_view(mp);
+  if (prune) {
+    bool any = false;
+    for (uint32_t c = 0; c < m.isz && !any; ++c)
+      any = old_children[(uint64_t)s * m.isz + c] < kPendingChild;
+    if (!any) return;
+  }
   const int32_t ns = root_insert(m, key);
   if (ns < 0) return;
   for (uint32_t c = 0; c < m.isz; ++c) {
     const uint32_t leaf = old_children[(uint64_t)s * m.isz + c];
     m.children[(uint64_t)ns * m.isz + c] = leaf;
     if (leaf < kPendingChild) m.origin[leaf].w = ns;
+  }
+}
+
+// ------------------------------------------ releaseUnusedMemory compaction
+
+// live[k] = 1 if leaf k has any cell on (bonxai.hpp:367-387 keeps exactly those)
+__global__ void __launch_bounds__(256) k_leaf_live(const MapDev* __restrict__ mp, uint32_t nleaves,
+                                                   uint32_t* __restrict__ live) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nleaves) return;
+  const MapView m = load_view(mp);
+  uint64_t any = 0;
+  for (uint32_t w = 0; w < m.lw; ++w) any |= m.masks[(uint64_t)k * m.lw + w];
+  live[k] = any != 0;
+}
+
+// Exclusive prefix sum of n 32-bit values in tiles of kScanTile: per-tile
+// sums, then a one-block scan of the tile sums, then each tile with its carry.
+__global__ void __launch_bounds__(1024) k_tile_reduce(const uint32_t* __restrict__ in, uint64_t n,
+                                                      uint32_t* __restrict__ tiles) {
+  __shared__ uint32_t wsum[16];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + 4ull * threadIdx.x;
+  uint32_t s = 0;
+  for (uint32_t j = 0; j < 4; ++j) s += base + j < n ? in[base + j] : 0u;
+  s = wave_scan(s);
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  if (lane == 63) wsum[wid] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (uint32_t w = 0; w < 16; ++w) t += wsum[w];
+    tiles[blockIdx.x] = t;
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_tile_sums_scan(uint32_t* __restrict__ tiles,
+                                                         uint32_t ntiles, uint32_t* __restrict__ total) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t carry;
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t b = 0; b < ntiles; b += 1024) {
+    const uint32_t i = b + threadIdx.x;
+    const uint32_t v = i < ntiles ? tiles[i] : 0u;
+    const uint32_t incl = wave_scan(v);
+    if (lane == 63) wsum[wid] = incl;
+    __syncthreads();
+    if (wid == 0) {
+      const uint32_t w = lane < 16u ? wsum[lane] : 0u;
+      const uint32_t wi = wave_scan(w);
+      if (lane < 16u) wsum[lane] = wi;
+    }
+    __syncthreads();
+    const uint32_t excl = carry + (wid ? wsum[wid - 1] : 0u) + incl - v;
+    if (i < ntiles) tiles[i] = excl;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += wsum[15];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(1024) k_tile_scan(const uint32_t* __restrict__ in, uint64_t n,
+                                                    const uint32_t* __restrict__ tiles,
+                                                    uint32_t* __restrict__ out) {
+  __shared__ uint32_t wsum[16];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile + 4ull * threadIdx.x;
+  uint32_t v[4], s = 0;
+  for (uint32_t j = 0; j < 4; ++j) {
+    v[j] = base + j < n ? in[base + j] : 0u;
+    s += v[j];
+  }
+  const uint32_t incl = wave_scan(s);
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  if (lane == 63) wsum[wid] = incl;
+  __syncthreads();
+  if (wid == 0) {
+    const uint32_t w = lane < 16u ? wsum[lane] : 0u;
+    const uint32_t wi = wave_scan(w);
+    if (lane < 16u) wsum[lane] = wi;
+  }
+  __syncthreads();
+  uint32_t run = tiles[blockIdx.x] + (wid ? wsum[wid - 1] : 0u) + incl - s;
+  for (uint32_t j = 0; j < 4; ++j) {
+    if (base + j < n) out[base + j] = run;
+    run += v[j];
+  }
+}
+
+// Move every live leaf k to index newidx[k] of the new pool arrays and point
+// its inner-grid entry there; the entries of dead leaves become empty.
+// One block per old leaf.
+__global__ void __launch_bounds__(256) k_compact_leaves(
+    const MapDev* __restrict__ mp, uint32_t nleaves, const uint32_t* __restrict__ live,
+    const uint32_t* __restrict__ newidx, uint64_t* __restrict__ dmasks, uint8_t* __restrict__ dcells,
+    int4* __restrict__ dorigin) {
+  const uint32_t k = blockIdx.x;
+  if (k >= nleaves) return;
+  const MapView m = load_view(mp);
+  const int4 o = m.origin[k];
+  const bool on = live[k] != 0;
+  const uint32_t nk = newidx[k];
+  if (threadIdx.x == 0) {
+    const uint32_t ii = inner_index(o.x, o.y, o.z, m.lb, m.ib);
+    m.children[(uint64_t)o.w * m.isz + ii] = on ? nk : kEmptyChild;
+    if (on) dorigin[nk] = o;
+  }
+  if (!on) return;
+  for (uint32_t w = threadIdx.x; w < m.lw; w += blockDim.x)
+    dmasks[(uint64_t)nk * m.lw + w] = m.masks[(uint64_t)k * m.lw + w];
+  const uint64_t nb = (uint64_t)m.lsz * m.vb;
+  if (nb % 4 == 0) {
+    const uint32_t BXG_G* src = (const uint32_t BXG_G*)(m.cells + (uint64_t)k * nb);
+    uint32_t* dst = (uint32_t*)(dcells + (uint64_t)nk * nb);
+    for (uint64_t b = threadIdx.x; b < nb / 4; b += blockDim.x) dst[b] = src[b];
+  } else {
+    for (uint64_t b = threadIdx.x; b < nb; b += blockDim.x)
+      dcells[(uint64_t)nk * nb + b] = m.cells[(uint64_t)k * nb + b];
   }
 }
 

@@ -10,6 +10,9 @@ container, where the reference tree and oracle/_ref exist):
                   (oracle/_ref) and checked against the restatement
   grid.npz        VoxelGrid<float> results of the reference's own VoxelGrid
                   on a reduced clustered workload (was_on, lookups, cells)
+  serial_*.bin    grid streams written by the reference's own Serialize
+                  (serialization.hpp:77-118): the examples/test_serialization.cpp
+                  walk (VoxelGrid<int>, 0.1 m) and 8k clustered float cells
   digests.json    input digests of every synthetic workload and order-free
                   map digests of the full-size workloads (restatement,
                   cross-checked against the reference's VoxelGrid where that
@@ -126,6 +129,35 @@ def make_grid():
     print(f"grid: {len(c)} cells")
 
 
+def serial_cube_coords(res=0.1):
+    """The walk of the reference's examples/test_serialization.cpp:16-24:
+    x, y, z from -0.5 by += res while < 0.5, posToCoord of each, in order."""
+    inv = 1.0 / res
+    steps = []
+    x = -0.5
+    while x < 0.5:
+        steps.append(int(np.floor(x * inv)))
+        x += res
+    coords = [(a, b, c) for a in steps for b in steps for c in steps]
+    return np.array(coords, np.int32), np.arange(len(coords), dtype=np.int32)
+
+
+def make_serial():
+    """Streams written by the reference's own Serialize (oracle/_ref)."""
+    coords, vals = serial_cube_coords()
+    r = oracle.RefGrid(0.1, np.int32)
+    r.set_values(coords, vals)
+    with open(os.path.join(HERE, "serial_cube_int.bin"), "wb") as f:
+        f.write(r.serialize())
+    print(f"serial cube: {r.active_cells_count()} cells")
+    c = scenes.clusters(n=8_000)
+    r = oracle.RefGrid(0.05, np.float32)
+    r.set_values(c, scenes.cluster_values(c))
+    with open(os.path.join(HERE, "serial_clusters_float.bin"), "wb") as f:
+        f.write(r.serialize())
+    print(f"serial clusters: {r.active_cells_count()} cells")
+
+
 def fnv(a):
     return scenes.fnv1a(np.ascontiguousarray(a))
 
@@ -210,6 +242,7 @@ def main():
         np.save(os.path.join(HERE, "apple_xyz.npy"), read_pcd_xyz(REF_DATA))
     make_kat()
     make_grid()
+    make_serial()
     d = make_digests(full)
     d["full"] = full
     with open(os.path.join(HERE, "digests.json"), "w") as f:

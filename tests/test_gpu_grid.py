@@ -131,6 +131,59 @@ def test_clear_options(ctx):
     og.close()
 
 
+@pytest.mark.parametrize("dtype", [np.float32, None])
+def test_release_unused_memory(ctx, dtype):
+    """releaseUnusedMemory (bonxai.hpp:367-387): leaves whose cells are all off
+    and roots left without leaves go; the rest (values included) stays, and
+    the map keeps working after the pools are compacted."""
+    import bonxai_amd as bx
+    import oracle
+    rng = np.random.default_rng(8)
+    a = rng.integers(-400, 400, (60_000, 3)).astype(np.int32)
+    v = rng.random(len(a)).astype(np.float32)
+    g = bx.VoxelGrid(ctx, 0.1, dtype=dtype)
+    og = oracle.Grid(0.1, dtype=dtype)
+    if dtype is None:
+        g.set_cells_on(a)
+        og.set_cells_on(a)
+    else:
+        g.set_values(a, v)
+        og.set_values(a, v)
+    # every cell of a third of the leaves off, plus scattered cells elsewhere
+    leaf = a & ~7
+    drop = (np.abs(leaf).sum(1) // 8) % 3 == 0
+    off = np.concatenate([a[drop], a[~drop][::7]])
+    assert np.array_equal(g.set_cells_off(off, True), og.set_cells_off(off))
+    before = g.stats()
+    g.release_unused_memory()
+    og.release_unused()
+    st = g.stats()
+    assert st["leaf_count"] == og.leaf_count() < before["leaf_count"]
+    assert st["root_count"] == og.root_count() <= before["root_count"]
+    assert st["leaf_capacity"] < before["leaf_capacity"]
+    assert_same_cells(g.export(), og.export(), "after releaseUnusedMemory")
+    assert g.digest() == og.digest()
+    # new leaves and values into the compacted pools
+    b = rng.integers(-600, 600, (20_000, 3)).astype(np.int32)
+    w = rng.random(len(b)).astype(np.float32)
+    if dtype is None:
+        assert np.array_equal(g.set_cells_on(b, None, True), og.set_cells_on(b))
+    else:
+        assert np.array_equal(g.set_values(b, w, True), og.set_values(b, w))
+    assert_same_cells(g.export(), og.export(), "after refill")
+    vals, found = g.lookup(a)
+    ovals, ofound = og.lookup(a)
+    assert np.array_equal(found, ofound)
+    if dtype is not None:
+        assert np.array_equal(vals.view(np.uint32), ovals.view(np.uint32))
+    # releasing everything
+    g.clear(bx.SET_ALL_CELLS_OFF)
+    g.release_unused_memory()
+    assert g.stats()["leaf_count"] == 0 and g.stats()["root_count"] == 0
+    g.close()
+    og.close()
+
+
 def test_leaf_paging_round_trip(ctx):
     """Whole-leaf read/write (the host accessor's paging path)."""
     import bonxai_amd as bx
