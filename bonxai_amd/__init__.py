@@ -82,6 +82,21 @@ class ScanDesc(ctypes.Structure):
                 ("origin", ctypes.c_double * 3), ("max_range", ctypes.c_double)]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("max_inflight", ctypes.c_uint32), ("max_batch_scans", ctypes.c_uint32),
+                ("streams", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("kind", ctypes.c_uint32), ("count", ctypes.c_uint64),
+                ("submit_ns", ctypes.c_uint64), ("admit_ns", ctypes.c_uint64), ("done_ns", ctypes.c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("submitted", "completed", "failed", "retried", "deferred",
+                                               "waves", "wave_scans", "queued", "inflight")]
+
+
 _PPTR = ctypes.POINTER(ctypes.c_void_p)
 
 ABI = [
@@ -147,6 +162,15 @@ ABI = [
     ("bxg_graph_destroy", _INT, [_P]),
     ("bxg_fp64_probe", _INT, [_P, _P, _P, _P, _U64, _P]),
     ("bxg_copy_probe", _INT, [_P, _P, _P, _U64]),
+    ("bxg_server_create", _INT, [_P, ctypes.POINTER(ServerOptions), _PPTR]),
+    ("bxg_server_destroy", _INT, [_P]),
+    ("bxg_server_drain", _INT, [_P]),
+    ("bxg_server_insert", _INT, [_P, _P, _P, _U64, _U32, _PD, ctypes.c_double, _PU64]),
+    ("bxg_server_lookup", _INT, [_P, _P, _P, _U64, _P, _P, _PU64]),
+    ("bxg_server_export", _INT, [_P, _P, _INT, _P, _P, _U64, _PU64]),
+    ("bxg_server_wait", _INT, [_P, _U64, ctypes.POINTER(RequestInfo)]),
+    ("bxg_server_query", _INT, [_P, _U64, ctypes.POINTER(RequestInfo)]),
+    ("bxg_server_get_stats", _INT, [_P, ctypes.POINTER(ServerStats)]),
 ]
 
 _lib = None
@@ -624,4 +648,76 @@ class ScanGraph:
     def close(self):
         if self._h:
             lib().bxg_graph_destroy(self._h)
+            self._h = None
+
+
+class Server:
+    """The serving engine (bxg_server_*): maps stay resident, requests are
+    admitted continuously by in-flight slots and pool memory, requests of one
+    map run in submission order, requests of different maps overlap.  Inputs
+    and outputs are DeviceArrays that must live until the request is waited
+    for."""
+
+    KINDS = ("insert", "lookup", "export")
+
+    def __init__(self, ctx, max_inflight=32, max_batch_scans=16, streams=4):
+        self.ctx = ctx
+        o = ServerOptions(max_inflight, max_batch_scans, streams, 0)
+        h = ctypes.c_void_p()
+        _check(lib().bxg_server_create(ctx._h, ctypes.byref(o), ctypes.byref(h)), ctx, "server")
+        self._h = h.value
+
+    def insert(self, m, points, origin, max_range):
+        if not isinstance(points, DeviceArray):
+            raise TypeError("server requests take DeviceArray inputs")
+        flags = POINTS_F64 if points.dtype == np.float64 else 0
+        o = (ctypes.c_double * 3)(*[float(x) for x in origin])
+        rid = ctypes.c_uint64()
+        _check(lib().bxg_server_insert(self._h, m._h, points.ptr, points.shape[0], flags, o,
+                                       float(max_range), ctypes.byref(rid)), self.ctx, "server insert")
+        return rid.value
+
+    def lookup(self, m, coords, vals=None, found=None):
+        rid = ctypes.c_uint64()
+        _check(lib().bxg_server_lookup(self._h, m._h, coords.ptr, coords.shape[0],
+                                       vals.ptr if vals is not None else None,
+                                       found.ptr if found is not None else None, ctypes.byref(rid)),
+               self.ctx, "server lookup")
+        return rid.value
+
+    def export(self, m, coords, vals=None, filter=EXPORT_ALL):
+        rid = ctypes.c_uint64()
+        _check(lib().bxg_server_export(self._h, m._h, filter, coords.ptr,
+                                       vals.ptr if vals is not None else None, coords.shape[0],
+                                       ctypes.byref(rid)), self.ctx, "server export")
+        return rid.value
+
+    @staticmethod
+    def _info(i):
+        return {"status": i.status, "kind": Server.KINDS[i.kind], "count": i.count,
+                "submit_ns": i.submit_ns, "admit_ns": i.admit_ns, "done_ns": i.done_ns,
+                "latency_ms": (i.done_ns - i.submit_ns) / 1e6 if i.done_ns else None}
+
+    def wait(self, rid):
+        """Block until request rid is done; its info dict (status 0 = ok)."""
+        i = RequestInfo()
+        lib().bxg_server_wait(self._h, rid, ctypes.byref(i))
+        return self._info(i)
+
+    def query(self, rid):
+        i = RequestInfo()
+        _check(lib().bxg_server_query(self._h, rid, ctypes.byref(i)), self.ctx, "server query")
+        return self._info(i)
+
+    def drain(self):
+        _check(lib().bxg_server_drain(self._h), self.ctx, "server drain")
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().bxg_server_get_stats(self._h, ctypes.byref(s)), self.ctx, "server stats")
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            lib().bxg_server_destroy(self._h)
             self._h = None

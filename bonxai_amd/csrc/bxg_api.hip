@@ -169,25 +169,32 @@ hipEvent_t ev_take(bxg_ctx* c) {
   return e;
 }
 
-// launch on the context stream, bracketed by events when profiling
+// launch on a stream of the context, bracketed by events when profiling
 template <typename... KArgs, typename... Args>
-int launch(bxg_ctx* c, const char* name, void (*k)(KArgs...), uint32_t grid, uint32_t block,
-           Args... args) {
+int launch_on(bxg_ctx* c, hipStream_t stream, const char* name, void (*k)(KArgs...), uint32_t grid,
+              uint32_t block, Args... args) {
   if (grid == 0) return BXG_OK;
   hipEvent_t a = nullptr;
   if (c->profiling) {
     a = ev_take(c);
-    hipEventRecord(a, c->stream);
+    hipEventRecord(a, stream);
   }
-  hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, c->stream, args...);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(block), 0, stream, args...);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(c, e, name);
   if (c->profiling) {
     hipEvent_t b = ev_take(c);
-    hipEventRecord(b, c->stream);
+    hipEventRecord(b, stream);
     c->prof_pending.push_back({name, a, b});
   }
   return BXG_OK;
+}
+
+// launch on the context stream
+template <typename... KArgs, typename... Args>
+int launch(bxg_ctx* c, const char* name, void (*k)(KArgs...), uint32_t grid, uint32_t block,
+           Args... args) {
+  return launch_on(c, c->stream, name, k, grid, block, args...);
 }
 
 int dmalloc(bxg_ctx* c, void** p, uint64_t bytes) {
@@ -1859,3 +1866,5 @@ int bxg_fp64_probe(bxg_ctx* c, const double* a, const double* b, const double* c
 }
 
 }  // extern "C"
+
+#include "bxg_server.hip"

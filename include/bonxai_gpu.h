@@ -244,6 +244,61 @@ int bxg_graph_set_scans(bxg_graph* g, const bxg_scan_desc* scans, uint32_t nscan
 int bxg_graph_launch(bxg_graph* g);
 int bxg_graph_destroy(bxg_graph* g);
 
+/* ------------------------------------------------------------ serving */
+
+/* A request engine over maps resident on one context (no reference
+ * counterpart: the reference's callers integrate one scan at a time,
+ * bonxai_server.cpp:178-181).  Requests of one map run in submission order,
+ * one at a time, so each sees exactly the requests submitted before it on
+ * that map; requests of different maps overlap on the GPU (inserts of
+ * several maps form one scan wave, lookups and exports run on the engine's
+ * own streams beside it).  An insert is admitted when its map's node pools
+ * can grow to the scan's head-room estimate within the context's memory
+ * budget, else it waits for in-flight work (BXG_ERR_BUDGET if there is
+ * none).  All buffers are device memory owned by the caller and must stay
+ * valid until the request is waited for.  While a server runs, its maps
+ * must be used through the server only. */
+typedef struct bxg_server bxg_server;
+typedef struct bxg_server_options {
+  uint32_t max_inflight;    /* requests on the GPU at once (default 32) */
+  uint32_t max_batch_scans; /* inserts merged into one scan wave (default 16, at most 64) */
+  uint32_t streams;         /* streams for lookups and exports (default 4) */
+  uint32_t reserved;
+} bxg_server_options;
+typedef struct bxg_request_info {
+  int32_t status;     /* BXG_OK or an error; -1 while pending */
+  uint32_t kind;      /* 0 insert, 1 lookup, 2 export */
+  uint64_t count;     /* export: matching cells */
+  uint64_t submit_ns; /* steady-clock times of submission, launch, completion */
+  uint64_t admit_ns;
+  uint64_t done_ns;
+} bxg_request_info;
+typedef struct bxg_server_stats {
+  uint64_t submitted, completed, failed, retried, deferred;
+  uint64_t waves, wave_scans; /* scan waves launched, scans in them */
+  uint64_t queued, inflight;
+} bxg_server_stats;
+
+int bxg_server_create(bxg_ctx* ctx, const bxg_server_options* opts, bxg_server** out);
+/* waits for every request, then stops the engine */
+int bxg_server_destroy(bxg_server* s);
+int bxg_server_drain(bxg_server* s);
+/* ProbabilisticMap::insertPointCloud (probabilistic_map.hpp:141-160) */
+int bxg_server_insert(bxg_server* s, bxg_map* map, const void* points_dev, uint64_t n, uint32_t flags,
+                      const double origin[3], double max_range, uint64_t* id);
+/* ConstAccessor::value / isCellOn over a batch (bonxai.hpp:496-534) */
+int bxg_server_lookup(bxg_server* s, bxg_map* map, const int32_t* coords_dev, uint64_t n, void* vals_dev,
+                      uint8_t* found_dev, uint64_t* id);
+/* forEachCell / getOccupiedVoxels / getFreeVoxels (bonxai.hpp:704-743,
+ * probabilistic_map.cpp:108-126): filter BXG_EXPORT_*; count in the info */
+int bxg_server_export(bxg_server* s, bxg_map* map, int filter, int32_t* coords_dev, void* vals_dev,
+                      uint64_t cap, uint64_t* id);
+/* block until done; returns the request's status and forgets the id */
+int bxg_server_wait(bxg_server* s, uint64_t id, bxg_request_info* info);
+/* non-blocking: status -1 while pending */
+int bxg_server_query(bxg_server* s, uint64_t id, bxg_request_info* info);
+int bxg_server_get_stats(bxg_server* s, bxg_server_stats* out);
+
 /* ---------------------------------------------------------- checks */
 
 /* a plain device-to-device copy kernel (16 bytes a lane), bytes % 16 == 0:
