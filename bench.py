@@ -14,7 +14,9 @@ update each frame, no cell is frozen).
 Prints one JSON line.  Extra fields: the plain per-scan rate, the
 batched/plain ratio, voxel updates/s, per-frame p50/p99 latency, the
 roofline of the ray kernel (k_scan_rays) and of the bulk lookup kernel
-(VoxelGrid<float>, 2M inserts / 8M lookups), and the CPU baseline.
+(VoxelGrid<float>, 2M inserts / 8M lookups), the serving form (16 resident
+maps, a Poisson stream of depth-frame inserts, lookups and exports; p50/p99
+per request type), and the CPU baseline.
 """
 import argparse
 import json
@@ -182,6 +184,83 @@ def run_lookups(ctx, bx, scenes, reps=20):
     }
 
 
+def run_serving(ctx, bx, scenes, nreq=800, rate=1500.0, nmaps=16, seed=21):
+    """Serving form: nmaps resident maps at 0.05 m, a Poisson stream of
+    requests -- inserts of 640x480 depth frames (max range 5 m), lookup
+    batches log-uniform in 10k..1M coords, occupied-voxel exports -- admitted
+    by the engine while earlier ones run.  Latency = completion - submission."""
+    rng = np.random.default_rng(seed)
+    maps = [bx.ProbabilisticMap(ctx, 0.05) for _ in range(nmaps)]
+    frames = []
+    for k in range(24):
+        pts, origin = scenes.depth_frame(1000 + k)
+        frames.append((ctx.to_device(pts), origin))
+    big = [ctx.to_device(np.column_stack([rng.integers(-160, 160, 1_000_000), rng.integers(-160, 160, 1_000_000),
+                                          rng.integers(-20, 80, 1_000_000)]).astype(np.int32)) for _ in range(4)]
+    nout = 2 * 32
+    outs = [(ctx.array(1_000_000, np.uint32), ctx.array(1_000_000, np.uint8)) for _ in range(nout)]
+    ex = [(ctx.array((8_000_000, 3), np.int32), ctx.array(8_000_000, np.uint32)) for _ in range(4)]
+    ctx.sync()
+    srv = bx.Server(ctx, max_inflight=32, max_batch_scans=16, streams=4)
+    # resident maps: two frames each before the stream starts
+    warm = [srv.insert(m, *frames[(2 * i + j) % len(frames)], 5.0) for i, m in enumerate(maps) for j in range(2)]
+    for w in warm:
+        if srv.wait(w)["status"] != 0:
+            raise RuntimeError("serving warm-up insert failed")
+    kinds = rng.choice(3, nreq, p=[0.5, 0.4, 0.1])
+    which = rng.integers(0, nmaps, nreq)
+    sizes = np.exp(rng.uniform(np.log(10_000), np.log(1_000_000), nreq)).astype(np.int64)
+    gaps = rng.exponential(1.0 / rate, nreq)
+    ids = []
+    t0 = time.perf_counter()
+    t_next = t0
+    for k in range(nreq):
+        t_next += gaps[k]
+        while True:
+            dt = t_next - time.perf_counter()
+            if dt <= 0:
+                break
+            if dt > 3e-4:
+                time.sleep(dt - 2e-4)
+        m = maps[which[k]]
+        if kinds[k] == 0:
+            d, o = frames[k % len(frames)]
+            ids.append(("insert", srv.insert(m, d, o, 5.0), d.shape[0]))
+        elif kinds[k] == 1:
+            v, f = outs[k % nout]
+            ids.append(("lookup", srv.lookup(m, big[k % 4].view(int(sizes[k])), v, f), int(sizes[k])))
+        else:
+            c, v = ex[k % 4]
+            ids.append(("export", srv.export(m, c, v, filter=bx.EXPORT_OCCUPIED), 0))
+    t_sub = time.perf_counter() - t0
+    infos = [(kind, srv.wait(i), n) for kind, i, n in ids]
+    t_all = time.perf_counter() - t0
+    stats = srv.stats()
+    srv.close()
+    if any(i["status"] != 0 for _, i, _ in infos):
+        raise RuntimeError("a serving request failed")
+    first = min(i["submit_ns"] for _, i, _ in infos)
+    last = max(i["done_ns"] for _, i, _ in infos)
+    out = {"requests": nreq, "offered_rate_per_s": rate, "maps": nmaps, "submit_s": t_sub,
+           "wall_s": t_all, "requests_per_s": nreq / ((last - first) / 1e9),
+           "waves": stats["waves"], "scans_per_wave": stats["wave_scans"] / max(stats["waves"], 1),
+           "deferred": stats["deferred"], "retried": stats["retried"]}
+    for kind in ("insert", "lookup", "export"):
+        lat = np.array([i["latency_ms"] for kd, i, _ in infos if kd == kind])
+        work = sum(n for kd, _, n in infos if kd == kind)
+        if len(lat):
+            out[kind] = {"count": int(len(lat)), "p50_ms": float(np.percentile(lat, 50)),
+                         "p99_ms": float(np.percentile(lat, 99)),
+                         "items_per_s": work / ((last - first) / 1e9)}
+    for d, _ in frames:
+        d.free()
+    for a in big + [x for pair in outs + ex for x in pair]:
+        a.free()
+    for m in maps:
+        m.close()
+    return out
+
+
 def cpu_baseline(scans, max_range, budget_s):
     import oracle
     t0 = time.perf_counter()
@@ -208,6 +287,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-lookups", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=800)
+    ap.add_argument("--serve-rate", type=float, default=1500.0, help="offered requests per second")
     ap.add_argument("--ray-blocks-per-cu", type=int, default=0,
                     help="override the tuned workgroups per CU of the ray kernels")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -215,8 +297,8 @@ def main():
                          "reference for scaling HBM byte counters) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU: every figure is for one MI355X "
-                                   "(see DESIGN.md for how the path would shard)",
+        print(json.dumps({"error": "this build runs on one GPU: every figure is for one MI355X; "
+                                   "multi-GPU sharding is designed but not built",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
@@ -233,7 +315,7 @@ def main():
     t0 = time.time()
     scans = scenes.batch_scans(nmaps=args.maps, max_range=max_range)
     log(f"generated {len(scans)} scans, {sum(len(p) for p, _ in scans)} points in {time.time() - t0:.1f}s")
-    ctx = bx.Context(0)
+    ctx = bx.Context(0, tuned=True)
     if args.ray_blocks_per_cu:
         ctx.set_tuning("ray_blocks_per_cu", args.ray_blocks_per_cu)
     dev = [ctx.to_device(p) for p, _ in scans]
@@ -248,6 +330,11 @@ def main():
     if not args.no_lookups:
         lk = run_lookups(ctx, bx, scenes)
         log(f"lookups: {lk['lookup_ms']:.3f} ms per 8M")
+    sv = None
+    if not args.no_serving:
+        sv = run_serving(ctx, bx, scenes, nreq=args.serve_requests, rate=args.serve_rate)
+        log(f"serving: {sv['requests_per_s']:.0f} req/s, insert p50/p99 "
+            f"{sv['insert']['p50_ms']:.2f}/{sv['insert']['p99_ms']:.2f} ms")
 
     npts = b["points"]
     ms_step = b["total_ms"] / args.steps
@@ -261,6 +348,8 @@ def main():
     k_ms = b["kernels_ms"]["k_scan_rays"]
     alg_bytes = 20 * b["rays_traced"] + 8 * b["ray_updates"]
     achieved = alg_bytes / (k_ms / 1e3) / 1e9
+    from bonxai_amd import autotune
+    ctx_tuning = {k: ctx.get_tuning(k) for k in autotune.CANDIDATES}
     cpu_rate, cpu_maps, cpu_pts, cpu_dt = cpu_baseline(scans, max_range, args.cpu_seconds)
     lat = np.array(b["lat"])
     out = {
@@ -282,6 +371,7 @@ def main():
                         "one captured graph per frame",
             "maps": len(scans), "points_per_frame": npts, "voxel": 0.1, "max_range": max_range,
             "rays_traced_per_frame": b["rays_traced"], "ray_updates_per_frame": b["ray_updates"],
+            "tuning": {k: ctx_tuning[k] for k in sorted(ctx_tuning)},
         },
         "voxel_updates_per_s": upd_s,
         "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
@@ -311,6 +401,8 @@ def main():
                                             "unit": "GB/s", "frac": lk_gbs / HBM_PEAK_GBS,
                                             "traffic": measured_traffic("k_lookup"),
                                             "bytes_per_lookup": 28}}
+    if sv is not None:
+        out["serving"] = sv
     for d in dev:
         d.free()
     ctx.close()

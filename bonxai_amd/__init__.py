@@ -78,8 +78,9 @@ class MapStats(ctypes.Structure):
 
 class ScanDesc(ctypes.Structure):
     _fields_ = [("map", ctypes.c_void_p), ("points_dev", ctypes.c_void_p), ("n", ctypes.c_uint64),
-                ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
-                ("origin", ctypes.c_double * 3), ("max_range", ctypes.c_double)]
+                ("flags", ctypes.c_uint32), ("stride", ctypes.c_uint32),
+                ("origin", ctypes.c_double * 3), ("max_range", ctypes.c_double),
+                ("sensor_to_world", ctypes.c_void_p)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -154,6 +155,8 @@ ABI = [
     ("bxg_insert_point_cloud_host", _INT, [_P, _P, _U64, _U32, _PD, _D]),
     ("bxg_insert_batch", _INT, [_P, ctypes.POINTER(ScanDesc), _U32]),
     ("bxg_add_point_host", _INT, [_P, _PD, _INT]),
+    ("bxg_insert_point_cloud_tf", _INT, [_P, _P, _U64, _U32, _P, ctypes.c_double]),
+    ("bxg_insert_point_cloud_tf_host", _INT, [_P, _P, _U64, _U32, _P, ctypes.c_double]),
     ("bxg_classify", _INT, [_P, _P, _U64, _P]),
     ("bxg_classify_host", _INT, [_P, _P, _U64, _P]),
     ("bxg_graph_create", _INT, [_P, _U32, _U64, _PPTR]),
@@ -246,24 +249,42 @@ class DeviceArray:
         return out
 
     def free(self):
-        if self.ptr:
+        if self.ptr and not getattr(self, "_view", False):
             lib().bxg_free(self.ctx._h, self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    def view(self, count):
+        """The first `count` rows, sharing this allocation (not owning it)."""
+        if count > self.shape[0]:
+            raise ValueError("view longer than the array")
+        v = DeviceArray.__new__(DeviceArray)
+        v.ctx, v.dtype, v.ptr, v._view = self.ctx, self.dtype, self.ptr, True
+        v.shape = (int(count),) + self.shape[1:]
+        v.nbytes = int(np.prod(v.shape)) * self.dtype.itemsize
+        return v
 
     def __len__(self):
         return self.shape[0]
 
 
 class Context:
-    """One GPU: stream, scratch, memory budget (bytes, 0 = unlimited)."""
+    """One GPU: stream, scratch, memory budget (bytes, 0 = unlimited).  With
+    tuned=True the launch setting recorded for this device by
+    bonxai_amd.autotune (bonxai_amd/tuning.json) is applied, if there is one."""
 
-    def __init__(self, device=0, budget_bytes=0):
+    def __init__(self, device=0, budget_bytes=0, tuned=False):
         h = ctypes.c_void_p()
         rc = lib().bxg_ctx_create(device, budget_bytes, ctypes.byref(h))
         if rc != OK:
             raise BonxaiError(rc, "context creation failed on device %d: %s"
                               % (device, lib().bxg_status_string(rc).decode()))
         self._h = h.value
+        self.tuning = None
+        if tuned:
+            from . import autotune
+            self.tuning = autotune.load(self)
+            if self.tuning:
+                autotune.apply(self, self.tuning)
 
     def close(self):
         if self._h:
@@ -568,6 +589,28 @@ class ProbabilisticMap(VoxelGrid):
         _check(lib().bxg_insert_point_cloud_host(self._h, _ptr(p), len(p), flags, o, float(max_range)),
                self.ctx, "insert_point_cloud")
 
+    def insert_point_cloud_tf(self, points, sensor_to_world, max_range, sync=True):
+        """The reference server's ingest pre-step fused into the scan
+        (bonxai_server.cpp:147-179): float points in the sensor frame, (n, k)
+        with x y z first (k = 4 is pcl::PointXYZ's layout), non-finite points
+        dropped, the rest moved by the 4x4 float sensor-to-world matrix, origin
+        = the matrix's translation."""
+        t = np.ascontiguousarray(sensor_to_world, np.float32).reshape(16)
+        if isinstance(points, DeviceArray):
+            if points.dtype != np.float32:
+                raise ValueError("the ingest transform takes float points")
+            stride = int(np.prod(points.shape[1:])) * 4
+            _check(lib().bxg_insert_point_cloud_tf(self._h, points.ptr, len(points), stride, _ptr(t),
+                                                   float(max_range)), self.ctx, "insert_point_cloud_tf")
+            if sync:
+                _check(self.sync(), self.ctx, "insert_point_cloud_tf")
+            return
+        p = np.ascontiguousarray(points, np.float32)
+        if p.ndim != 2 or p.shape[1] < 3:
+            raise ValueError("points must be (n, k >= 3) float32")
+        _check(lib().bxg_insert_point_cloud_tf_host(self._h, _ptr(p), len(p), p.shape[1] * 4, _ptr(t),
+                                                    float(max_range)), self.ctx, "insert_point_cloud_tf")
+
     def add_hit_point(self, xyz):
         p = (ctypes.c_double * 3)(*[float(x) for x in xyz])
         _check(lib().bxg_add_point_host(self._h, p, 1), self.ctx, "add_hit_point")
@@ -605,17 +648,26 @@ class ProbabilisticMap(VoxelGrid):
 
 
 def _scan_descs(scans):
+    """scans: (map, DeviceArray points, origin, max_range) or (map, DeviceArray
+    float points (n, k >= 3), None, max_range, sensor_to_world 4x4) tuples."""
     arr = (ScanDesc * len(scans))()
     keep = []
-    for i, (m, pts, origin, max_range) in enumerate(scans):
+    for i, sc in enumerate(scans):
+        m, pts, origin, max_range = sc[:4]
         if not isinstance(pts, DeviceArray):
             raise TypeError("batched scans take DeviceArray points")
         arr[i].map = m._h
         arr[i].points_dev = pts.ptr
         arr[i].n = len(pts)
         arr[i].flags = POINTS_F64 if pts.dtype == np.float64 else 0
-        for a in range(3):
-            arr[i].origin[a] = float(origin[a])
+        arr[i].stride = int(np.prod(pts.shape[1:])) * pts.dtype.itemsize
+        if len(sc) > 4 and sc[4] is not None:
+            t = np.ascontiguousarray(sc[4], np.float32).reshape(16)
+            arr[i].sensor_to_world = t.ctypes.data
+            keep.append(t)
+        else:
+            for a in range(3):
+                arr[i].origin[a] = float(origin[a])
         arr[i].max_range = float(max_range)
         keep.append(pts)
     return arr, keep

@@ -1,0 +1,143 @@
+"""Autotuner of the batched scan-integration launch.
+
+Sweeps, on a sample frame of uneven scans into many maps (captured-graph
+launches, as the batched form runs them):
+  ray_block           workgroup size of the two work-stealing ray walks
+  ray_blocks_per_cu   workgroups per CU of the cell-update walk
+  leaf_blocks_per_cu  workgroups per CU of the leaf-allocation walk
+  ray_chunk           most rays a wavefront takes per cursor atomic
+                      (rays per lane = ray_chunk / 64)
+  ep_table_factor     endpoint-table slots per point (inverse load factor)
+  root_slack          root-directory slots per root (inverse load factor)
+by coordinate descent from the defaults, and records the fastest setting per
+device (architecture name + CU count) in bonxai_amd/tuning.json.
+Context(tuned=True) applies the recorded setting of its device.
+
+Usage: python -m bonxai_amd.autotune [--maps 32] [--passes 2] [--out PATH]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNING_FILE = os.path.join(HERE, "tuning.json")
+
+CANDIDATES = {
+    "ray_block": [64, 128, 256],
+    "ray_blocks_per_cu": [2, 4, 8, 16],
+    "leaf_blocks_per_cu": [4, 8, 16, 32],
+    "ray_chunk": [64, 128, 256, 512],
+    "ep_table_factor": [2, 4],
+    "root_slack": [2, 4],
+}
+
+
+def device_key(ctx):
+    info = ctx.device_info()
+    return f"{info['arch']}|{info['cu_count']}"
+
+
+def load(ctx, path=TUNING_FILE):
+    """The recorded setting of ctx's device, or None."""
+    try:
+        with open(path) as f:
+            return json.load(f).get(device_key(ctx), {}).get("settings")
+    except (OSError, ValueError):
+        return None
+
+
+def apply(ctx, settings):
+    for k, v in settings.items():
+        ctx.set_tuning(k, int(v))
+
+
+def frame_ms(ctx, bx, descs, npoints, reps=5, warmup=2):
+    """Milliseconds per frame of a graph captured with the current tuning."""
+    g = bx.ScanGraph(ctx, len(descs), npoints)
+    g.set_scans(descs)
+    for _ in range(warmup):
+        g.launch()
+    ctx.sync()
+    ctx.event(0)
+    for _ in range(reps):
+        g.launch()
+    ctx.event(1)
+    ms = ctx.elapsed_ms(0, 1) / reps
+    g.close()
+    return ms
+
+
+def search(ctx, bx, scans, passes=2, log=print):
+    max_range = 30.0
+    maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+    dev = [ctx.to_device(p) for p, _ in scans]
+    descs = [(m, d, o, max_range) for m, d, (_, o) in zip(maps, dev, scans)]
+    npoints = sum(len(p) for p, _ in scans)
+    bx.insert_batch(ctx, descs)  # sizes the pools
+    for m in maps:
+        st = m.stats()
+        m.reserve(int(st["leaf_count"] * 1.25) + 8192, int(st["root_count"] * 2) + 1024)
+    best = {k: ctx.get_tuning(k) for k in CANDIDATES}
+    apply(ctx, best)
+    default_ms = best_ms = frame_ms(ctx, bx, descs, npoints)
+    table = [(dict(best), best_ms)]
+    log(f"default {best} {best_ms:.3f} ms")
+    for p in range(passes):
+        for key, values in CANDIDATES.items():
+            for v in values:
+                if v == best[key]:
+                    continue
+                trial = dict(best, **{key: v})
+                apply(ctx, trial)
+                ms = frame_ms(ctx, bx, descs, npoints)
+                table.append((trial, ms))
+                log(f"pass {p} {key}={v}: {ms:.3f} ms")
+                if ms < best_ms * 0.99:
+                    best, best_ms = trial, ms
+            apply(ctx, best)
+    for m in maps:
+        if m.sync() != 0:
+            raise RuntimeError("a map failed during tuning")
+        m.close()
+    for d in dev:
+        d.free()
+    return best, best_ms, default_ms, table
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description="autotune the batched scan launch")
+    ap.add_argument("--maps", type=int, default=32)
+    ap.add_argument("--passes", type=int, default=2)
+    ap.add_argument("--out", default=TUNING_FILE)
+    args = ap.parse_args(argv)
+    sys.path.insert(0, os.path.dirname(HERE))
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    scans = scenes.batch_scans(nmaps=args.maps, max_range=30.0)
+    ctx = bx.Context(0)
+    t0 = time.time()
+    best, best_ms, default_ms, table = search(ctx, bx, scans, args.passes,
+                                              log=lambda *a: print(*a, flush=True))
+    rec = {}
+    try:
+        with open(args.out) as f:
+            rec = json.load(f)
+    except (OSError, ValueError):
+        pass
+    rec[device_key(ctx)] = {
+        "settings": best, "frame_ms": best_ms, "default_frame_ms": default_ms,
+        "workload": f"{len(scans)} LiDAR scans into {len(scans)} maps, 0.1 m, 30 m, "
+                    f"{sum(len(p) for p, _ in scans)} points, captured graph",
+        "evaluations": len(table), "seconds": time.time() - t0,
+    }
+    with open(args.out, "w") as f:
+        json.dump(rec, f, indent=1, sort_keys=True)
+    print(json.dumps(rec[device_key(ctx)]), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

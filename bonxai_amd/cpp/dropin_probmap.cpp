@@ -14,6 +14,8 @@
 //   3 addMissPoint           double xyz[3]
 //   4 query                  uint32 n, int32 xyz[3n]
 //   5 accessor write         uint32 n, int32 xyz[3n], int32 word[n]  (value(coord, true) = word)
+//   6 insert, sensor frame   uint32 n, float sensor_to_world[16] (row-major), double max_range,
+//                            float xyzw[4n] (a pcl::PointXYZ-like point: x y z pad)
 // result: per query op, per coordinate: int8 isOccupied, isFree, isUnknown,
 // classify state, int64 cell word through a const accessor (-1 if off); then
 // uint64 n + n x (int32 xyz, int32 word) from forEachCell, uint64 n + n x int32
@@ -31,8 +33,11 @@
 
 namespace {
 
-struct FloatPoint {  // a pcl::PointXYZ-like type with float members
+struct FloatPoint {  // a float point type with x, y, z members
   float x, y, z;
+};
+struct PaddedPoint {  // pcl::PointXYZ's layout: x y z and a pad word
+  float x, y, z, pad;
 };
 
 struct Reader {
@@ -139,6 +144,14 @@ int main(int argc, char** argv) {
           c->update_id = int32_t(uint32_t(words[i]) << 28) >> 28;
           c->probability_log = words[i] >> 4;
         }
+      } else if (kind == 6) {
+        const uint32_t n = r.get<uint32_t>();
+        float tf[16];
+        r.get(tf, 16);
+        const double range = r.get<double>();
+        std::vector<PaddedPoint> pts(n);
+        r.get(reinterpret_cast<float*>(pts.data()), 4 * size_t(n));
+        map.insertPointCloudTransformed(pts, tf, range);
       } else {
         throw std::runtime_error("unknown op");
       }

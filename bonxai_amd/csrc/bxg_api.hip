@@ -81,9 +81,16 @@ struct bxg_ctx {
   bool profiling = false;
   bool counting = false;
   // tuning (bxg_ctx_set_tuning): workgroups per CU of the work-stealing
-  // update pass and leaf-allocation pass over the rays
+  // update pass and leaf-allocation pass over the rays, the workgroup size
+  // of both, the most rays a wavefront takes per cursor atomic (rays per
+  // lane = ray_chunk / 64), the endpoint-table capacity per point and the
+  // root-directory capacity per root (inverse load factors)
   uint32_t ray_blocks_per_cu = 4;
   uint32_t leaf_blocks_per_cu = 16;
+  uint32_t ray_block = 256;
+  uint32_t ray_chunk = 256;
+  uint32_t ep_table_factor = 2;
+  uint32_t root_slack = 2;
   std::vector<Prof> prof_pending;
   std::vector<hipEvent_t> ev_free;
   std::map<std::string, std::pair<uint64_t, double>> prof_totals;
@@ -437,7 +444,8 @@ int ensure_room(bxg_map* m, uint64_t leaves, uint64_t roots) {
   const uint64_t want_leaves = m->known_leaves + leaves;
   if (want_leaves > m->leaf_cap * 3 / 4) BXG_TRY(grow_leaves(m, want_leaves * 3 / 2 + 1024));
   const uint64_t want_roots = m->known_roots + roots;
-  if (want_roots * 2 > m->root_cap) BXG_TRY(grow_roots(m, want_roots * 2));
+  const uint64_t slack = m->ctx->root_slack;
+  if (want_roots * slack > m->root_cap) BXG_TRY(grow_roots(m, want_roots * slack));
   return BXG_OK;
 }
 
@@ -459,7 +467,7 @@ int ensure_scan_scratch(bxg_ctx* c, uint64_t points, uint32_t nscans) {
     c->sc.offs = offs;
     const uint64_t cap = std::max<uint64_t>(pow2_at_least(points), 1 << 12);
     if (cap > (1ull << 31)) return fail(c, BXG_ERR_INVALID_ARG, "too many points in one call");
-    const uint64_t tcap = cap * 2;
+    const uint64_t tcap = cap * c->ep_table_factor;
     void* p;
     BXG_TRY(dmalloc(c, &p, cap * sizeof(int4)));
     c->sc.rec = (int4*)p;
@@ -574,14 +582,27 @@ int build_descs(bxg_ctx* c, const bxg_scan_desc* scans, uint32_t nscans, uint32_
     d.points = in.points_dev;
     d.pt_off = off;
     d.n = (uint32_t)in.n;
-    d.flags = ((in.flags & BXG_POINTS_F64) ? kScanF64 : 0u) | extra_flags;
-    d.fx = in.origin[0];
-    d.fy = in.origin[1];
-    d.fz = in.origin[2];
+    const bool f64 = (in.flags & BXG_POINTS_F64) != 0;
+    d.flags = (f64 ? kScanF64 : 0u) | extra_flags;
+    const uint32_t packed = f64 ? 24u : 12u;
+    d.stride = in.stride ? in.stride : packed;
+    if (d.stride < packed || d.stride % (f64 ? 8u : 4u))
+      return fail(c, BXG_ERR_INVALID_ARG, "point stride smaller than a point or misaligned");
+    double origin[3] = {in.origin[0], in.origin[1], in.origin[2]};
+    if (in.sensor_to_world) {
+      if (f64) return fail(c, BXG_ERR_INVALID_ARG, "the ingest transform takes float points");
+      d.flags |= kScanTransform;
+      std::memcpy(d.tf, in.sensor_to_world, sizeof(d.tf));
+      // the sensor position: the matrix's translation (bonxai_server.cpp:172-175)
+      for (int k = 0; k < 3; ++k) origin[k] = (double)in.sensor_to_world[4 * k + 3];
+    }
+    d.fx = origin[0];
+    d.fy = origin[1];
+    d.fz = origin[2];
     d.max_range = in.max_range;
     d.r2 = in.max_range * in.max_range;  // probabilistic_map.hpp:145
     int32_t o[3];
-    BXG_TRY(origin_voxel(m, in.origin, o));
+    BXG_TRY(origin_voxel(m, origin, o));
     d.ox = o[0];
     d.oy = o[1];
     d.oz = o[2];
@@ -624,24 +645,26 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
   // than the rays could use (64 rays per wavefront, 4 wavefronts a block)
   const uint32_t cus = (uint32_t)std::max(c->cu_count, 1);
   // at most one lane per point (the winners are fewer)
-  const uint32_t most = (uint32_t)((grid_points + 255) / 256);
+  const uint32_t rb = c->ray_block;
+  const uint32_t most = (uint32_t)((grid_points + rb - 1) / rb);
   const uint32_t gr = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->ray_blocks_per_cu, most));
   const uint32_t gl = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->leaf_blocks_per_cu, most));
+  sc.chunk_max = c->ray_chunk;
   BXG_TRY(launch(c, "k_scan_classify", k_scan_classify, gp, kBlock, hdr, desc, sc));
   BXG_TRY(launch(c, "k_scan_winners", k_scan_winners, gp, kBlock, hdr, desc, sc));
   BXG_TRY(launch(c, "k_scan_bucket_reduce", k_scan_bucket_reduce, kTiles, 1024, hdr, sc));
   BXG_TRY(launch(c, "k_scan_bucket_scan", k_scan_bucket_scan, kTiles, 1024, hdr, sc));
   BXG_TRY(launch(c, "k_scan_scatter", k_scan_scatter, gp, kBlock, hdr, sc));
-  BXG_TRY(launch(c, "k_scan_ray_leaves", k_scan_ray_leaves, gl, kBlock, desc, sc));
+  BXG_TRY(launch(c, "k_scan_ray_leaves", k_scan_ray_leaves, gl, rb, desc, sc));
   if (any_extra)
     BXG_TRY(launch(c, "k_scan_extra_leaves", k_scan_extra_leaves, nscans, kBlock, hdr, desc));
   BXG_TRY(launch(c, "k_scan_endpoints", k_scan_endpoints, gw, kBlock, desc, sc));
   if (c->counting) {
-    BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<true>, gr, kBlock, desc, sc));
+    BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<true>, gr, rb, desc, sc));
     if (any_extra)
       BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<true>, nscans, kBlock, hdr, desc));
   } else {
-    BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<false>, gr, kBlock, desc, sc));
+    BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<false>, gr, rb, desc, sc));
     if (any_extra)
       BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<false>, nscans, kBlock, hdr, desc));
   }
@@ -1000,29 +1023,58 @@ int bxg_profile_reset(bxg_ctx* c) {
   c->prof_totals.clear();
   return BXG_OK;
 }
+}  // extern "C"
+
+namespace {
+
+// the tuning keys: name, field, smallest and largest value, required divisor
+struct TuningKey {
+  const char* name;
+  uint32_t bxg_ctx::*field;
+  int64_t lo, hi, step;
+};
+const TuningKey kTuning[] = {
+    {"ray_blocks_per_cu", &bxg_ctx::ray_blocks_per_cu, 1, 256, 1},
+    {"leaf_blocks_per_cu", &bxg_ctx::leaf_blocks_per_cu, 1, 256, 1},
+    {"ray_block", &bxg_ctx::ray_block, 64, 256, 64},
+    {"ray_chunk", &bxg_ctx::ray_chunk, 64, 1024, 64},
+    {"ep_table_factor", &bxg_ctx::ep_table_factor, 2, 8, 1},
+    {"root_slack", &bxg_ctx::root_slack, 2, 8, 1},
+};
+
+const TuningKey* tuning_key(const char* key) {
+  for (const TuningKey& k : kTuning)
+    if (!std::strcmp(key, k.name)) return &k;
+  return nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
 int bxg_ctx_set_tuning(bxg_ctx* c, const char* key, int64_t value) {
   if (!c || !key) return BXG_ERR_INVALID_ARG;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
-  uint32_t* slot = nullptr;
-  if (!std::strcmp(key, "ray_blocks_per_cu")) slot = &c->ray_blocks_per_cu;
-  if (!std::strcmp(key, "leaf_blocks_per_cu")) slot = &c->leaf_blocks_per_cu;
-  if (!slot) return fail(c, BXG_ERR_INVALID_ARG, std::string("unknown tuning key ") + key);
-  if (value < 1 || value > 256) return fail(c, BXG_ERR_INVALID_ARG, std::string(key) + " in 1..256");
-  *slot = (uint32_t)value;
+  const TuningKey* k = tuning_key(key);
+  if (!k) return fail(c, BXG_ERR_INVALID_ARG, std::string("unknown tuning key ") + key);
+  if (value < k->lo || value > k->hi || value % k->step)
+    return fail(c, BXG_ERR_INVALID_ARG, std::string(key) + ": value out of range");
+  if (k->field == &bxg_ctx::ep_table_factor && (value & (value - 1)))
+    return fail(c, BXG_ERR_INVALID_ARG, "ep_table_factor must be a power of two");
+  if (k->field == &bxg_ctx::ep_table_factor && (uint32_t)value != c->ep_table_factor) {
+    BXG_TRY(ctx_sync(c));
+    c->scan_cap = 0;  // the endpoint table is rebuilt at the next scan
+  }
+  c->*(k->field) = (uint32_t)value;
   return BXG_OK;
 }
 
 int bxg_ctx_get_tuning(bxg_ctx* c, const char* key, int64_t* value) {
   if (!c || !key || !value) return BXG_ERR_INVALID_ARG;
-  if (!std::strcmp(key, "ray_blocks_per_cu")) {
-    *value = c->ray_blocks_per_cu;
-    return BXG_OK;
-  }
-  if (!std::strcmp(key, "leaf_blocks_per_cu")) {
-    *value = c->leaf_blocks_per_cu;
-    return BXG_OK;
-  }
-  return fail(c, BXG_ERR_INVALID_ARG, std::string("unknown tuning key ") + key);
+  const TuningKey* k = tuning_key(key);
+  if (!k) return fail(c, BXG_ERR_INVALID_ARG, std::string("unknown tuning key ") + key);
+  *value = c->*(k->field);
+  return BXG_OK;
 }
 
 int bxg_set_counting(bxg_ctx* c, int on) {
@@ -1572,19 +1624,13 @@ int bxg_set_update_count(bxg_map* m, uint32_t v) {
   return ctx_sync(m->ctx);
 }
 
-int bxg_insert_point_cloud(bxg_map* m, const void* points, uint64_t n, uint32_t flags,
-                           const double origin[3], double max_range) {
-  if (!m || !origin) return BXG_ERR_INVALID_ARG;
+}  // extern "C"
+
+namespace {
+
+// one scan, enqueued (the device-pointer insert entry points)
+int insert_async(bxg_map* m, const bxg_scan_desc& s) {
   bxg_ctx* c = m->ctx;
-  std::lock_guard<std::recursive_mutex> lk(c->mu);
-  bxg_scan_desc s;
-  std::memset(&s, 0, sizeof(s));
-  s.map = m;
-  s.points_dev = points;
-  s.n = n;
-  s.flags = flags;
-  std::memcpy(s.origin, origin, sizeof(s.origin));
-  s.max_range = max_range;
   BXG_TRY(run_batch(c, &s, 1, 0));
   // rays due from addHit/MissPoint are traced by this insert
   if (!m->extra.empty()) {
@@ -1592,6 +1638,65 @@ int bxg_insert_point_cloud(bxg_map* m, const void* points, uint64_t n, uint32_t 
     m->extra.clear();
   }
   return BXG_OK;
+}
+
+// one scan of host points: staged, run, re-run after growing the pools if
+// the map ran out of nodes (a failed run wrote no cell)
+int insert_host(bxg_map* m, bxg_scan_desc s, const void* points) {
+  bxg_ctx* c = m->ctx;
+  BXG_TRY(map_failed(m));
+  const uint64_t packed = (s.flags & BXG_POINTS_F64) ? 24 : 12;
+  const uint64_t bytes = s.n ? (s.n - 1) * (s.stride ? s.stride : packed) + packed : 0;
+  void* dp;
+  BXG_TRY(stage_in(c, 3, points, bytes, &dp));
+  s.points_dev = dp;
+  for (int attempt = 0;; ++attempt) {
+    BXG_TRY(run_batch(c, &s, 1, 0));
+    BXG_TRY(read_counters(m));
+    const uint32_t st = m->h_ctr->status;
+    if (!st) break;
+    if (st & (kStCoordRange | kStScratchFull)) {
+      BXG_TRY(clear_status(m));
+      return fail(c, status_code(st), "insertPointCloud failed: coordinate outside the supported range");
+    }
+    if (attempt > 8) return fail(c, BXG_ERR_OUT_OF_POOL, "could not grow the node pool");
+    BXG_TRY(grow_after_overflow(m));
+  }
+  m->extra.clear();
+  return BXG_OK;
+}
+
+bxg_scan_desc scan_of(bxg_map* m, const void* points, uint64_t n, uint32_t flags, uint32_t stride,
+                      const double* origin, const float* tf, double max_range) {
+  bxg_scan_desc s;
+  std::memset(&s, 0, sizeof(s));
+  s.map = m;
+  s.points_dev = points;
+  s.n = n;
+  s.flags = flags;
+  s.stride = stride;
+  if (origin) std::memcpy(s.origin, origin, sizeof(s.origin));
+  s.sensor_to_world = tf;
+  s.max_range = max_range;
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bxg_insert_point_cloud(bxg_map* m, const void* points, uint64_t n, uint32_t flags,
+                           const double origin[3], double max_range) {
+  if (!m || !origin) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  return insert_async(m, scan_of(m, points, n, flags, 0, origin, nullptr, max_range));
+}
+
+int bxg_insert_point_cloud_tf(bxg_map* m, const float* points, uint64_t n, uint32_t stride,
+                              const float sensor_to_world[16], double max_range) {
+  if (!m || !sensor_to_world) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  return insert_async(m, scan_of(m, points, n, 0, stride, nullptr, sensor_to_world, max_range));
 }
 
 // Run one wave, then re-run (alone, after growing its pools) every scan
@@ -1642,35 +1747,15 @@ int bxg_insert_batch(bxg_ctx* c, const bxg_scan_desc* scans, uint32_t nscans) {
 int bxg_insert_point_cloud_host(bxg_map* m, const void* points, uint64_t n, uint32_t flags,
                                 const double origin[3], double max_range) {
   if (!m || !origin || (n && !points)) return BXG_ERR_INVALID_ARG;
-  bxg_ctx* c = m->ctx;
-  std::lock_guard<std::recursive_mutex> lk(c->mu);
-  BXG_TRY(map_failed(m));
-  const uint64_t bytes = n * ((flags & BXG_POINTS_F64) ? 24 : 12);
-  void* dp;
-  BXG_TRY(stage_in(c, 3, points, bytes, &dp));
-  bxg_scan_desc s;
-  std::memset(&s, 0, sizeof(s));
-  s.map = m;
-  s.points_dev = dp;
-  s.n = n;
-  s.flags = flags;
-  std::memcpy(s.origin, origin, sizeof(s.origin));
-  s.max_range = max_range;
-  for (int attempt = 0;; ++attempt) {
-    BXG_TRY(run_batch(c, &s, 1, 0));
-    BXG_TRY(read_counters(m));
-    const uint32_t st = m->h_ctr->status;
-    if (!st) break;
-    if (st & (kStCoordRange | kStScratchFull)) {
-      BXG_TRY(clear_status(m));
-      return fail(c, status_code(st), "insertPointCloud failed: coordinate outside the supported range");
-    }
-    if (attempt > 8) return fail(c, BXG_ERR_OUT_OF_POOL, "could not grow the node pool");
-    // nothing was written to any cell; grow and run the scan again
-    BXG_TRY(grow_after_overflow(m));
-  }
-  m->extra.clear();
-  return BXG_OK;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  return insert_host(m, scan_of(m, nullptr, n, flags, 0, origin, nullptr, max_range), points);
+}
+
+int bxg_insert_point_cloud_tf_host(bxg_map* m, const float* points, uint64_t n, uint32_t stride,
+                                   const float sensor_to_world[16], double max_range) {
+  if (!m || !sensor_to_world || (n && !points)) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  return insert_host(m, scan_of(m, nullptr, n, 0, stride, nullptr, sensor_to_world, max_range), points);
 }
 
 int bxg_add_point_host(bxg_map* m, const double xyz[3], int is_hit) {
@@ -1724,7 +1809,7 @@ int bxg_graph_create(bxg_ctx* c, uint32_t max_scans, uint64_t max_points, bxg_gr
   g->max_scans = max_scans;
   g->max_points = max_points;
   const uint64_t cap = pow2_at_least(std::max<uint64_t>(max_points, 4096));
-  const uint64_t tcap = cap * 2;
+  const uint64_t tcap = cap * c->ep_table_factor;
   void* p = nullptr;
   int rc = dmalloc(c, &p, cap * sizeof(int4));
   g->sc.rec = (int4*)p;

@@ -88,7 +88,8 @@ struct alignas(16) ScanDesc {
   uint32_t tag;        // scan index inside the batch (endpoint table key)
   const int4* extra;   // endpoints added by addHit/MissPoint whose rays are due
   uint32_t nextra;
-  uint32_t pad;
+  uint32_t stride;     // bytes from one point to the next
+  float tf[12];        // kScanTransform: rows 0..2 of the sensor-to-world matrix
 };
 
 enum : uint32_t {
@@ -96,6 +97,7 @@ enum : uint32_t {
   kScanForceHit = 2u,   // addHitPoint: no max-range clip, kind hit
   kScanForceMiss = 4u,  // addMissPoint: no max-range clip, kind miss
   kScanNoRays = 8u,     // endpoint update only, keep update_count
+  kScanTransform = 16u, // float points: drop non-finite, then x' = tf * (x, y, z, 1)
 };
 
 // size of a batch, read by the kernels from device memory so that a captured
@@ -128,6 +130,7 @@ struct ScanScratch {
   uint64_t* ep_keys;    // endpoint table keys
   uint32_t* ep_first;   // endpoint table: min point index
   uint32_t ep_mask;     // capacity - 1
+  uint32_t chunk_max;   // most rays a wavefront takes per cursor atomic (64..1024)
   uint32_t* winners;    // flattened point indices of per-voxel winners, bucketed
   uint32_t* wkey;       // per point: bucket of a winner, kNone otherwise
   uint32_t* hist;       // [kMaxBuckets] winners per bucket (zero between calls)

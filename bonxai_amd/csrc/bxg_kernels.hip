@@ -427,16 +427,33 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
   const MapView m = load_view(d.map);
   const uint64_t i = gi - d.pt_off;
   double tx, ty, tz;
+  const uint8_t BXG_G* pb = gp((const uint8_t*)d.points) + i * d.stride;
   if (d.flags & kScanF64) {
-    const double BXG_G* p = gp((const double*)d.points) + 3 * i;
+    const double BXG_G* p = (const double BXG_G*)pb;
     tx = p[0];
     ty = p[1];
     tz = p[2];
   } else {
-    const float BXG_G* p = gp((const float*)d.points) + 3 * i;
-    tx = (double)p[0];
-    ty = (double)p[1];
-    tz = (double)p[2];
+    const float BXG_G* p = (const float BXG_G*)pb;
+    float x = p[0], y = p[1], z = p[2];
+    if (d.flags & kScanTransform) {
+      // bonxai_server.cpp:147-153 drops points with a non-finite coordinate
+      if (!(__builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z))) {
+        sc.rec[gi] = make_int4(0, 0, 0, (int32_t)(s << 16));
+        return;
+      }
+      // pcl::transformPointCloud in float (bonxai_server.cpp:166-170), no contraction
+      const float* t = d.tf;
+      const float nx = t[0] * x + (t[1] * y + (t[2] * z + t[3]));
+      const float ny = t[4] * x + (t[5] * y + (t[6] * z + t[7]));
+      const float nz = t[8] * x + (t[9] * y + (t[10] * z + t[11]));
+      x = nx;
+      y = ny;
+      z = nz;
+    }
+    tx = (double)x;
+    ty = (double)y;
+    tz = (double)z;
   }
   const double vx = tx - d.fx, vy = ty - d.fy, vz = tz - d.fz;
   const double sq = (vx * vx + vy * vy) + vz * vz;
@@ -667,7 +684,7 @@ __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const
   return true;
 }
 
-constexpr uint32_t kChunk = 256;  // most rays a wavefront takes per cursor atomic
+constexpr uint32_t kChunk = 256;  // default of ScanScratch::chunk_max
 
 // Work-stealing walk over the winner list.  The list is split into kRegions
 // ranges; a wavefront starts in the range of its XCD (workgroups are
@@ -685,11 +702,12 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
   const uint32_t lane = lane_id();
   uint32_t chunk_pos = 0, chunk_end = 0;
   // small lists are one region: splitting them only multiplies cursor traffic
-  const uint32_t nreg = nwin >= kRegions * 8 * kChunk ? kRegions : 1u;
+  const uint32_t cmax = sc.chunk_max ? sc.chunk_max : kChunk;
+  const uint32_t nreg = nwin >= kRegions * 8 * cmax ? kRegions : 1u;
   // chunk size: kChunk rays when there is work for every wavefront, down
   // to one ray a lane on small lists
   uint32_t chunk = (nwin / (gridDim.x * (blockDim.x / 64u))) & ~63u;
-  chunk = chunk < 64u ? 64u : (chunk > kChunk ? kChunk : chunk);
+  chunk = chunk < 64u ? 64u : (chunk > cmax ? cmax : chunk);
   uint32_t region = blockIdx.x % nreg, tried = 0;
   bool exhausted = false;
   bool have = false;

@@ -85,12 +85,17 @@ typedef struct bxg_map_stats {
 /* one scan of a batched insert */
 typedef struct bxg_scan_desc {
   bxg_map* map;
-  const void* points_dev; /* xyz triples */
+  const void* points_dev; /* xyz triples (x, y, z first in every point) */
   uint64_t n;
   uint32_t flags;         /* BXG_POINTS_F64 */
-  uint32_t reserved;
-  double origin[3];
+  uint32_t stride;        /* bytes from one point to the next; 0 = packed xyz */
+  double origin[3];       /* ignored when sensor_to_world is set */
   double max_range;
+  /* NULL, or the ingest pre-step of bonxai_server.cpp:147-179 for float
+   * points: 16 floats, row-major sensor-to-world matrix; non-finite points are
+   * dropped, the others transformed, and the origin is the matrix's
+   * translation column */
+  const float* sensor_to_world;
 } bxg_scan_desc;
 
 typedef struct bxg_graph bxg_graph;
@@ -222,6 +227,19 @@ int bxg_insert_point_cloud(bxg_map* map, const void* points_dev, uint64_t n, uin
  * ran out of nodes (the failed run leaves every cell unchanged) */
 int bxg_insert_point_cloud_host(bxg_map* map, const void* points, uint64_t n, uint32_t flags,
                                 const double origin[3], double max_range);
+/* The ingest pre-step of the reference's server (bonxai_server.cpp:147-179)
+ * fused in front of the scan: float points in the sensor frame (stride bytes
+ * apart, x y z first; 0 = packed), points with a non-finite coordinate
+ * dropped, the others moved by the row-major sensor-to-world matrix in float
+ * (pcl::transformPointCloud: x' = m00 x + (m01 y + (m02 z + m03)), the
+ * association of PCL's SSE transform), then insertPointCloud(points,
+ * origin = (m03, m13, m23), max_range).  Enqueues only, like
+ * bxg_insert_point_cloud. */
+int bxg_insert_point_cloud_tf(bxg_map* map, const float* points_dev, uint64_t n, uint32_t stride,
+                              const float sensor_to_world[16], double max_range);
+/* same, host points; blocks and grows the pools as bxg_insert_point_cloud_host */
+int bxg_insert_point_cloud_tf_host(bxg_map* map, const float* points, uint64_t n, uint32_t stride,
+                                   const float sensor_to_world[16], double max_range);
 /* many scans in one pass over a flattened point queue; scans naming the
  * same map run in input order.  Blocks; a scan whose map ran out of nodes is
  * re-run after the pool grows (its failed run left the cells untouched). */

@@ -179,6 +179,35 @@ class ProbabilisticMap {
     _grid.refresh(true);
   }
 
+  /// The reference server's ingest pre-step fused into the scan
+  /// (bonxai_server.cpp:147-179): sensor-frame points with a non-finite
+  /// coordinate are dropped, the others moved by the float sensor-to-world
+  /// matrix (row-major 4x4) as pcl::transformPointCloud does, and the sensor
+  /// origin is the matrix's translation.  Equivalent to that filter, the
+  /// transform and insertPointCloud(points, PointXYZ(t), max_range), in one pass.
+  template <typename PointT, typename Allocator>
+  void insertPointCloudTransformed(const std::vector<PointT, Allocator>& sensor_points,
+                                   const float sensor_to_world[16], double max_range) {
+    _grid.flush();
+    _fbuf.resize(sensor_points.size() * 3);
+    for (size_t i = 0; i < sensor_points.size(); ++i)
+      for (int a = 0; a < 3; ++a) _fbuf[3 * i + a] = float(point_traits::get(sensor_points[i], a));
+    gpu::check(bxg_insert_point_cloud_tf_host(_grid.deviceMap(), _fbuf.data(), sensor_points.size(), 12,
+                                              sensor_to_world, max_range),
+               _grid.deviceContext());
+    _grid.refresh(true);
+  }
+#ifdef BONXAI_GPU_HAS_EIGEN
+  template <typename PointT, typename Allocator>
+  void insertPointCloudTransformed(const std::vector<PointT, Allocator>& sensor_points,
+                                   const Eigen::Matrix4f& sensor_to_world, double max_range) {
+    float m[16];
+    for (int r = 0; r < 4; ++r)
+      for (int c = 0; c < 4; ++c) m[4 * r + c] = sensor_to_world(r, c);
+    insertPointCloudTransformed(sensor_points, m, max_range);
+  }
+#endif
+
   /// The same for xyz triples already in GPU memory (float, or double with
   /// f64); enqueued on the map's stream without waiting when no host page is
   /// cached.

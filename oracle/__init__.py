@@ -60,6 +60,8 @@ def lib():
             "bxo_prob_add_miss": (None, [_P, _P]),
             "bxo_prob_update_free_cells": (None, [_P, _P]),
             "bxo_prob_insert_f32": (None, [_P, _P, _U64, _P, _D]),
+            "bxo_prob_insert_tf": (None, [_P, _P, _U64, _U64, _P, _D]),
+            "bxo_tf_ambiguity": (_U64, [_P, _U64, _U64, _P, _D, _D]),
             "bxo_prob_insert_f64": (None, [_P, _P, _U64, _P, _D]),
             "bxo_endpoints_f32": (_U64, [_P, _U64, _P, _D, _D, _P, _P]),
             "bxo_prob_classify": (None, [_P, _P, _U64, _P]),
@@ -230,6 +232,13 @@ class ProbMap:
             p = np.ascontiguousarray(p, np.float32)
             lib().bxo_prob_insert_f32(self.h, _p(p), len(p), _p(o), float(max_range))
 
+    def insert_point_cloud_tf(self, points, sensor_to_world, max_range):
+        """The ingest pre-step (finite filter, float transform, origin from the
+        matrix) then insertPointCloud; points (n, k >= 3) float32, x y z first."""
+        p = np.ascontiguousarray(points, np.float32)
+        t = np.ascontiguousarray(sensor_to_world, np.float32).reshape(16)
+        lib().bxo_prob_insert_tf(self.h, _p(p), len(p), p.shape[1], _p(t), float(max_range))
+
     def add_hit_point(self, xyz):
         lib().bxo_prob_add_hit(self.h, _p(np.ascontiguousarray(xyz, np.float64)))
 
@@ -269,6 +278,14 @@ def endpoints(points, origin, max_range, resolution):
     k = np.empty(len(p), np.uint8)
     amb = lib().bxo_endpoints_f32(_p(p), len(p), _p(o), float(max_range), 1.0 / resolution, _p(c), _p(k))
     return c, k, int(amb)
+
+
+def tf_ambiguity(points, sensor_to_world, max_range, resolution):
+    """Points whose endpoint differs between the two float associations of
+    the ingest transform."""
+    p = np.ascontiguousarray(points, np.float32)
+    t = np.ascontiguousarray(sensor_to_world, np.float32).reshape(16)
+    return int(lib().bxo_tf_ambiguity(_p(p), len(p), p.shape[1], _p(t), float(max_range), 1.0 / resolution))
 
 
 def compute_ray(o, e):

@@ -34,6 +34,9 @@
  *   isOccupied/isUnknown/isFree  probabilistic_map.cpp:56-75
  *   updateFreeCells         probabilistic_map.cpp:77-106
  *   getOccupied/FreeVoxels  probabilistic_map.cpp:108-126
+ *   ingest pre-step         bonxai_ros/src/bonxai_server.cpp:147-179 (NaN filter,
+ *                           float transform, origin) -- parity unpinned, see
+ *                           bxo_prob_insert_tf
  *
  * Eigen3 (absent from this image) supplies the arithmetic of
  * probabilistic_map.hpp:144-157.  Its squaredNorm() of a Vector3d is taken as
@@ -847,6 +850,67 @@ BXO_EXPORT uint64_t bxo_endpoints_f32(const float* pts, uint64_t n, const double
     ambiguous += (k != k2 || c[0] != c2[0] || c[1] != c2[1] || c[2] != c2[2]);
   }
   return ambiguous;
+}
+
+/* The ingest pre-step of bonxai_server.cpp:147-179 in front of
+ * insertPointCloud: points with a non-finite coordinate are dropped
+ * (:147-153, order kept), the others moved by the float sensor-to-world
+ * matrix (pcl::transformPointCloud, :166-170), and the sensor origin is the
+ * matrix's translation widened to double (:172-175).  PCL (absent from this
+ * image, version unpinned) computes x' = m00*x + (m01*y + (m02*z + m03)) in
+ * its SSE build and ((m00*x + m01*y) + m02*z) + m03 in its scalar build; the
+ * SSE association is taken (parity unpinned: bxo_tf_ambiguity counts the
+ * points whose endpoint differs between the two).  stride: floats from one
+ * point to the next. */
+static void bxo_tf_point(const float* t, const float* p, int alt, float* q) {
+  for (int r = 0; r < 3; ++r) {
+    const float* row = t + 4 * r;
+    q[r] = alt ? ((row[0] * p[0] + row[1] * p[1]) + row[2] * p[2]) + row[3]
+               : row[0] * p[0] + (row[1] * p[1] + (row[2] * p[2] + row[3]));
+  }
+}
+
+static uint64_t bxo_ingest(const float* pts, uint64_t n, uint64_t stride, const float m16[16], int alt,
+                           float* out) {
+  uint64_t k = 0;
+  for (uint64_t i = 0; i < n; ++i) {
+    const float* p = pts + i * stride;
+    if (!(isfinite(p[0]) && isfinite(p[1]) && isfinite(p[2]))) continue;
+    bxo_tf_point(m16, p, alt, out + 3 * k);
+    ++k;
+  }
+  return k;
+}
+
+BXO_EXPORT void bxo_prob_insert_tf(bxo_prob* m, const float* pts, uint64_t n, uint64_t stride,
+                                   const float m16[16], double max_range) {
+  float* q = (float*)malloc((n ? n : 1) * 3 * sizeof(float));
+  const uint64_t k = bxo_ingest(pts, n, stride, m16, 0, q);
+  const double origin[3] = {(double)m16[3], (double)m16[7], (double)m16[11]};
+  bxo_insert(m, q, 0, k, origin, max_range);
+  free(q);
+}
+
+/* Points whose endpoint voxel or hit/miss kind differs between the two
+ * associations of the float transform. */
+BXO_EXPORT uint64_t bxo_tf_ambiguity(const float* pts, uint64_t n, uint64_t stride, const float m16[16],
+                                     double max_range, double inv_resolution) {
+  float* a = (float*)malloc((n ? n : 1) * 3 * sizeof(float));
+  float* b = (float*)malloc((n ? n : 1) * 3 * sizeof(float));
+  const uint64_t k = bxo_ingest(pts, n, stride, m16, 0, a);
+  bxo_ingest(pts, n, stride, m16, 1, b);
+  const double fx = m16[3], fy = m16[7], fz = m16[11], r2 = max_range * max_range;
+  uint64_t diff = 0;
+  for (uint64_t i = 0; i < k; ++i) {
+    int32_t c[3], c2[3];
+    uint8_t ka, kb;
+    bxo_endpoint(fx, fy, fz, a[3 * i], a[3 * i + 1], a[3 * i + 2], max_range, r2, inv_resolution, 0, c, &ka);
+    bxo_endpoint(fx, fy, fz, b[3 * i], b[3 * i + 1], b[3 * i + 2], max_range, r2, inv_resolution, 0, c2, &kb);
+    diff += (ka != kb || c[0] != c2[0] || c[1] != c2[1] || c[2] != c2[2]);
+  }
+  free(a);
+  free(b);
+  return diff;
 }
 
 /* isOccupied / isUnknown / isFree, probabilistic_map.cpp:56-75:

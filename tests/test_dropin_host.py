@@ -93,3 +93,49 @@ def test_compute_ray_matches_oracle(checks):
         assert len(got) == int(np.abs(e - o).max())
         if len(got):
             assert np.array_equal(got[0], o) and not (got == e).all(1).any()
+
+
+@pytest.fixture(scope="module")
+def pcd_check(tmp_path_factory):
+    d = tmp_path_factory.mktemp("pcd")
+    exe = d / "dropin_pcd_check"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(ROOT, "tests", "dropin_pcd_check.cpp"), "-o", str(exe)], check=True)
+    return exe, d
+
+
+def _run_pcd(exe, d, src):
+    out = subprocess.run([str(exe), str(src), str(d / "pts.bin"), str(d / "copy.pcd")], check=True,
+                         stdout=subprocess.PIPE, text=True).stdout
+    return out, np.fromfile(d / "pts.bin", np.float64).reshape(-1, 3)
+
+
+def test_pcd_reads_reference_data(pcd_check):
+    """ReadPointsFromPCD (pcl_utils.cpp:10-26) on the reference's data/apple.pcd
+    (ascii, fields x y z rgb imX imY): every point, as float widened to double."""
+    exe, d = pcd_check
+    out, pts = _run_pcd(exe, d, os.path.join(ROOT, "tests", "golden", "apple.pcd"))
+    want = np.load(os.path.join(ROOT, "tests", "golden", "apple_xyz.npy"))
+    assert pts.shape == want.shape
+    assert np.array_equal(pts, want.astype(np.float64))
+    assert "reread equal 1" in out
+    assert "missing file: File doesn't exist" in out
+
+
+def test_pcd_reads_binary(pcd_check):
+    """A binary PCD with x y z among other fields (float32 and float64)."""
+    exe, d = pcd_check
+    rng = np.random.default_rng(1)
+    n = 500
+    rec = np.zeros(n, dtype=[("intensity", "<f4"), ("x", "<f4"), ("y", "<f4"), ("z", "<f8"), ("ring", "<u2")])
+    for k in ("x", "y", "z"):
+        rec[k] = rng.uniform(-50, 50, n)
+    hdr = ("# .PCD v0.7\nVERSION 0.7\nFIELDS intensity x y z ring\nSIZE 4 4 4 8 2\nTYPE F F F F U\n"
+           f"COUNT 1 1 1 1 1\nWIDTH {n}\nHEIGHT 1\nVIEWPOINT 0 0 0 1 0 0 0\nPOINTS {n}\nDATA binary\n")
+    src = d / "bin.pcd"
+    src.write_bytes(hdr.encode() + rec.tobytes())
+    out, pts = _run_pcd(exe, d, src)
+    want = np.stack([rec["x"].astype(np.float64), rec["y"].astype(np.float64),
+                     rec["z"].astype(np.float32).astype(np.float64)], 1)
+    assert np.array_equal(pts, want)
+    assert "reread equal 1" in out

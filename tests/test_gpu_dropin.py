@@ -207,6 +207,18 @@ def test_probabilistic_map_dropin(tmp_path):
     model.grid.set_values(wc, words.view(np.uint32))
     c, _ = scenes.box_scan(n=10_000, seed=6)
     insert(c, oa, 4.0, False)
+    # the server's ingest pre-step: sensor-frame PointXYZ-like points with
+    # non-finite entries, a float rigid transform, origin from the matrix
+    d, _ = scenes.box_scan(n=12_000, seed=8)
+    xyzw = np.zeros((len(d), 4), np.float32)
+    xyzw[:, :3] = d - d.mean(0)
+    xyzw[::97, 1] = np.nan
+    xyzw[5::131, 2] = np.inf
+    ang = 0.4
+    tf = np.array([[np.cos(ang), -np.sin(ang), 0, 0.35], [np.sin(ang), np.cos(ang), 0, -0.2],
+                   [0, 0, 1, 0.9], [0, 0, 0, 1]], np.float32)
+    sc.op(6, struct.pack("<I", len(xyzw)) + tf.tobytes() + struct.pack("<d", 6.0) + xyzw.tobytes())
+    model.insert_point_cloud_tf(xyzw, tf, 6.0)
     c2, _ = model.cells()
     query(query_coords(rng, c2))
 

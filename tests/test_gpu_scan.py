@@ -280,6 +280,47 @@ def test_batch_matches_sequential(ctx):
         d.free()
 
 
+@pytest.mark.parametrize("setting", [
+    {"ray_block": 64, "ray_chunk": 64, "ray_blocks_per_cu": 16, "leaf_blocks_per_cu": 4},
+    {"ray_block": 128, "ray_chunk": 512, "ep_table_factor": 4, "root_slack": 4},
+    {"ray_block": 256, "ray_chunk": 1024, "ray_blocks_per_cu": 1, "leaf_blocks_per_cu": 1},
+])
+def test_tuning_settings_keep_results(setting):
+    """Every launch setting the autotuner may pick (workgroup size, rays per
+    lane, workgroups per CU, table load factors) gives the same maps, in the
+    batched call and in a captured graph."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    c = bx.Context(0)
+    for k, v in setting.items():
+        c.set_tuning(k, v)
+        assert c.get_tuning(k) == v
+    scans = scenes.batch_scans(nmaps=6, lo=2_000, hi=30_000, seed=13)
+    maps = [bx.ProbabilisticMap(c, 0.1) for _ in scans]
+    dev = [c.to_device(p) for p, _ in scans]
+    descs = [(m, d, o, 30.0) for m, d, (_, o) in zip(maps, dev, scans)]
+    bx.insert_batch(c, descs)
+    g = bx.ScanGraph(c, len(scans), sum(len(p) for p, _ in scans))
+    g.set_scans(descs)
+    g.launch()
+    c.sync()
+    g.close()
+    for i, (m, (p, o)) in enumerate(zip(maps, scans)):
+        assert m.sync() == 0
+        om = ora.ProbMap(0.1)
+        om.insert_point_cloud(p, o, 30.0)
+        om.insert_point_cloud(p, o, 30.0)
+        assert_same_cells(_gpu_cells(m), om.cells(), f"map {i}")
+        assert m.update_count == om.update_count
+        om.close()
+    for m in maps:
+        m.close()
+    for d in dev:
+        d.free()
+    c.close()
+
+
 def test_batch_same_map_keeps_order(ctx):
     """Scans naming the same map in one batch run in input order."""
     import bonxai_amd as bx
