@@ -176,6 +176,11 @@ hipEvent_t ev_take(bxg_ctx* c) {
   return e;
 }
 
+// workgroups of k_lookup over n coordinates (kLookups a thread)
+uint32_t lookup_blocks(uint64_t n) {
+  return (uint32_t)((n + (uint64_t)kBlock * kLookups - 1) / ((uint64_t)kBlock * kLookups));
+}
+
 // launch on a stream of the context, bracketed by events when profiling
 template <typename... KArgs, typename... Args>
 int launch_on(bxg_ctx* c, hipStream_t stream, const char* name, void (*k)(KArgs...), uint32_t grid,
@@ -1387,7 +1392,7 @@ int bxg_lookup(bxg_map* m, const int32_t* coords, uint64_t n, void* vals, uint8_
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
   BXG_TRY(check_coords_args(m, coords, n));
   if (!n) return BXG_OK;
-  return launch(m->ctx, "k_lookup", k_lookup, nblocks(n), kBlock, (const MapDev*)m->d, coords, n,
+  return launch(m->ctx, "k_lookup", k_lookup, lookup_blocks(n), kBlock, (const MapDev*)m->d, coords, n,
                 (uint8_t*)vals, found);
 }
 int bxg_classify(bxg_map* m, const int32_t* coords, uint64_t n, int8_t* state) {
@@ -1481,7 +1486,7 @@ int bxg_lookup_host(bxg_map* m, const int32_t* coords, uint64_t n, void* vals, u
   BXG_TRY(stage_in(c, 0, coords, n * 12, &dc));
   if (vals && vb) BXG_TRY(stage_in(c, 1, nullptr, n * vb, &dv));
   if (found) BXG_TRY(stage_in(c, 2, nullptr, n, &df));
-  BXG_TRY(launch(c, "k_lookup", k_lookup, nblocks(n), kBlock, (const MapDev*)m->d,
+  BXG_TRY(launch(c, "k_lookup", k_lookup, lookup_blocks(n), kBlock, (const MapDev*)m->d,
                  (const int32_t*)dc, n, (uint8_t*)dv, (uint8_t*)df));
   if (dv) BXG_HIP(c, hipMemcpyAsync(vals, dv, n * vb, hipMemcpyDeviceToHost, c->stream));
   if (df) BXG_HIP(c, hipMemcpyAsync(found, df, n, hipMemcpyDeviceToHost, c->stream));

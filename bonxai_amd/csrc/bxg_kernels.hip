@@ -1047,26 +1047,77 @@ __global__ void __launch_bounds__(256) k_op_release(uint64_t n, OpScratch os) {
   }
 }
 
-// ConstAccessor::value / isCellOn (bonxai.hpp:496-534) for a batch
+// ConstAccessor::value / isCellOn (bonxai.hpp:496-534) for a batch.  Each
+// thread carries kLookups lookups through the chain root key -> inner entry
+// -> mask word -> cell together, so every stage issues kLookups independent
+// loads (lookup j of a block is element base + j * blockDim.x: every stage's
+// loads and stores stay coalesced across the wavefront).
+constexpr uint32_t kLookups = 4;
+
+__device__ __forceinline__ int32_t root_find_from(const MapView& m, uint64_t key, uint32_t h) {
+  for (uint32_t p = 0; p <= m.root_mask; ++p) {
+    const uint64_t k = m.root_keys[h];
+    if (k == key) return (int32_t)h;
+    if (k == kEmptyKey) return -1;
+    h = (h + 1) & m.root_mask;
+  }
+  return -1;
+}
+
 __global__ void __launch_bounds__(256) k_lookup(const MapDev* __restrict__ mp,
                                                 const int32_t* __restrict__ coords, uint64_t n,
                                                 uint8_t* __restrict__ vals,
                                                 uint8_t* __restrict__ found) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
   const MapView m = load_view(mp);
-  const int32_t x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
-  const uint32_t leaf = find_leaf(m, x, y, z);
-  const uint32_t vb = m.vb;
-  bool on = false;
-  uint64_t g = 0;
-  if (leaf != kNone) {
-    const uint32_t li = leaf_index(x, y, z, m.lb);
-    g = (uint64_t)leaf * m.lsz + li;
-    on = (m.masks[(uint64_t)leaf * m.lw + (li >> 6)] >> (li & 63)) & 1;
+  const uint64_t base = (uint64_t)blockIdx.x * blockDim.x * kLookups + threadIdx.x;
+  const int32_t BXG_G* cg = gp(coords);
+  int32_t x[kLookups], y[kLookups], z[kLookups];
+  uint64_t key[kLookups];
+  uint32_t h[kLookups];
+  bool live[kLookups];
+#pragma unroll
+  for (uint32_t j = 0; j < kLookups; ++j) {
+    const uint64_t i = base + (uint64_t)j * blockDim.x;
+    live[j] = i < n;
+    x[j] = live[j] ? cg[3 * i] : 0;
+    y[j] = live[j] ? cg[3 * i + 1] : 0;
+    z[j] = live[j] ? cg[3 * i + 2] : 0;
   }
-  if (found) found[i] = on;
-  if (vals && vb) {
+  uint64_t k0[kLookups];
+#pragma unroll
+  for (uint32_t j = 0; j < kLookups; ++j) {
+    live[j] = live[j] && pack_root(x[j], y[j], z[j], m.log2n, &key[j]);
+    h[j] = live[j] ? root_hash(key[j], m.root_mask) : 0u;
+    k0[j] = live[j] ? m.root_keys[h[j]] : kEmptyKey;
+  }
+  uint32_t c[kLookups];
+#pragma unroll
+  for (uint32_t j = 0; j < kLookups; ++j) {
+    int32_t slot = -1;
+    if (k0[j] == key[j] && live[j]) {
+      slot = (int32_t)h[j];
+    } else if (live[j] && k0[j] != kEmptyKey) {
+      slot = root_find_from(m, key[j], (h[j] + 1) & m.root_mask);  // a collision: probe on
+    }
+    c[j] = slot >= 0 ? m.children[(uint64_t)slot * m.isz + inner_index(x[j], y[j], z[j], m.lb, m.ib)]
+                     : kEmptyChild;
+  }
+  uint64_t mw[kLookups];
+  uint32_t li[kLookups];
+#pragma unroll
+  for (uint32_t j = 0; j < kLookups; ++j) {
+    li[j] = leaf_index(x[j], y[j], z[j], m.lb);
+    mw[j] = c[j] < kPendingChild ? m.masks[(uint64_t)c[j] * m.lw + (li[j] >> 6)] : 0ull;
+  }
+  const uint32_t vb = m.vb;
+#pragma unroll
+  for (uint32_t j = 0; j < kLookups; ++j) {
+    const uint64_t i = base + (uint64_t)j * blockDim.x;
+    const bool on = (mw[j] >> (li[j] & 63)) & 1;
+    const uint64_t g = (uint64_t)c[j] * m.lsz + li[j];
+    if (i >= n) continue;
+    if (found) found[i] = on;
+    if (!vals || !vb) continue;
     if (vb == 4) {
       ((uint32_t*)vals)[i] = on ? ((const uint32_t BXG_G*)m.cells)[g] : 0u;
     } else if (vb == 8) {

@@ -124,7 +124,7 @@ void srv_launch_read(bxg_server* s, SrvReq* r) {
     hipStream_t st = s->streams[s->next_stream++ % s->streams.size()];
     r->t_admit = now_ns();
     if (r->kind == kReqLookup) {
-      rc = launch_on(c, st, "k_lookup", k_lookup, nblocks(r->n), kBlock, (const MapDev*)m->d, r->coords,
+      rc = launch_on(c, st, "k_lookup", k_lookup, lookup_blocks(r->n), kBlock, (const MapDev*)m->d, r->coords,
                      r->n, (uint8_t*)r->vals, r->found);
     } else {
       // the map is idle: its last counter snapshot has landed or is about to

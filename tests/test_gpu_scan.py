@@ -94,6 +94,66 @@ def test_lidar_sequence_prefix(ctx):
     om.close()
 
 
+def test_lidar_sequence_against_digests(ctx):
+    """Sequential LiDAR mapping into one map (64 x 1024 rays a scan, 0.1 m,
+    50 m max range, 1 m/scan): the map's order-free digest and the scan's
+    endpoint/ray statistics after every scan equal the oracle's, committed in
+    tests/golden/digests.json (all 200 scans when the full set was generated,
+    else its first scans)."""
+    import json
+    import os
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    with open(os.path.join(os.path.dirname(__file__), "golden", "digests.json")) as f:
+        want = json.load(f)["maps"]["lidar_sequence"]
+    m = bx.ProbabilisticMap(ctx, 0.1)
+    ctx.counting(True)
+    try:
+        for k, (pts, origin) in enumerate(scenes.lidar_sequence(len(want))):
+            m.insert_point_cloud(pts, origin, 50.0)
+            got = [str(x) for x in m.digest()]
+            assert got == want[k]["digest"], f"scan {k}"
+            st = m.stats()
+            assert st["last_rays"] == want[k]["stats"]["rays"], f"scan {k} rays"
+            assert st["last_ray_updates"] == want[k]["stats"]["ray_updates"], f"scan {k} ray updates"
+    finally:
+        ctx.counting(False)
+    print(f"{len(want)} scans, {m.stats()['leaf_count']} leaves", flush=True)
+    m.close()
+
+
+def test_batched_frame_against_digests(ctx):
+    """The batched workload (64 maps, one scan each, log-uniform 10k..500k
+    points, 0.1 m, 30 m) in one batched pass: every map's digest and scan
+    statistics equal the oracle's in tests/golden/digests.json (all 64 maps
+    when the full set was generated, else the first ones)."""
+    import json
+    import os
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    with open(os.path.join(os.path.dirname(__file__), "golden", "digests.json")) as f:
+        want = json.load(f)["maps"]["batch"]
+    scans = scenes.batch_scans()
+    maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+    dev = [ctx.to_device(p) for p, _ in scans]
+    ctx.counting(True)
+    try:
+        bx.insert_batch(ctx, [(m, d, o, 30.0) for m, d, (_, o) in zip(maps, dev, scans)])
+        for w in want:
+            m = maps[w["map"]]
+            assert m.sync() == 0
+            assert [str(x) for x in m.digest()] == w["digest"], f"map {w['map']}"
+            st = m.stats()
+            assert st["last_rays"] == w["stats"]["rays"] and st["last_ray_updates"] == w["stats"]["ray_updates"]
+    finally:
+        ctx.counting(False)
+    print(f"{len(want)} maps checked", flush=True)
+    for m in maps:
+        m.close()
+    for d in dev:
+        d.free()
+
+
 def test_id_cycle_and_frozen_cells(ctx):
     """Eight small scans from alternating origins exercise the 3-value
     update_id cycle, including cells frozen because their id equals the
