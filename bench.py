@@ -45,6 +45,9 @@ def measured_traffic(kernel):
 METRIC = "rays/s + voxel updates/s, batched vs plain per-scan on 1 MI355X; p50/p99 latency"
 
 
+LOOKUP_BYTES = 21
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -395,12 +398,15 @@ def main():
                                             "p99": float(np.percentile(p["lat"], 99))}}
         out["batched_over_plain"] = rays_s / plain_rays_s
     if lk is not None:
-        lk_bytes = 28 * lk["lookups"]
+        # algorithmic bytes of a lookup: its 12-byte coordinate and 4-byte cell
+        # read, its 4-byte value and 1-byte flag written (a mask word is read
+        # only for a zero cell; root and inner-node reads are shared)
+        lk_bytes = LOOKUP_BYTES * lk["lookups"]
         lk_gbs = lk_bytes / (lk["lookup_ms"] / 1e3) / 1e9
         out["lookup"] = {**lk, "roofline": {"bound": "hbm", "achieved": lk_gbs, "peak": HBM_PEAK_GBS,
                                             "unit": "GB/s", "frac": lk_gbs / HBM_PEAK_GBS,
                                             "traffic": measured_traffic("k_lookup"),
-                                            "bytes_per_lookup": 28}}
+                                            "bytes_per_lookup": LOOKUP_BYTES}}
     if sv is not None:
         out["serving"] = sv
     for d in dev:

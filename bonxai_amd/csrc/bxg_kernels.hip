@@ -1049,9 +1049,9 @@ __global__ void __launch_bounds__(256) k_op_release(uint64_t n, OpScratch os) {
 
 // ConstAccessor::value / isCellOn (bonxai.hpp:496-534) for a batch.  Each
 // thread carries kLookups lookups through the chain root key -> inner entry
-// -> mask word -> cell together, so every stage issues kLookups independent
-// loads (lookup j of a block is element base + j * blockDim.x: every stage's
-// loads and stores stay coalesced across the wavefront).
+// -> cell (-> mask word) together, so every stage issues kLookups
+// independent loads (lookup j of a block is element base + j * blockDim.x:
+// every stage's loads and stores stay coalesced across the wavefront).
 constexpr uint32_t kLookups = 4;
 
 __device__ __forceinline__ int32_t root_find_from(const MapView& m, uint64_t key, uint32_t h) {
@@ -1102,28 +1102,45 @@ __global__ void __launch_bounds__(256) k_lookup(const MapDev* __restrict__ mp,
     c[j] = slot >= 0 ? m.children[(uint64_t)slot * m.isz + inner_index(x[j], y[j], z[j], m.lb, m.ib)]
                      : kEmptyChild;
   }
-  uint64_t mw[kLookups];
   uint32_t li[kLookups];
+  uint64_t g[kLookups];
 #pragma unroll
   for (uint32_t j = 0; j < kLookups; ++j) {
     li[j] = leaf_index(x[j], y[j], z[j], m.lb);
-    mw[j] = c[j] < kPendingChild ? m.masks[(uint64_t)c[j] * m.lw + (li[j] >> 6)] : 0ull;
+    g[j] = (uint64_t)(c[j] < kPendingChild ? c[j] : 0u) * m.lsz + li[j];
   }
   const uint32_t vb = m.vb;
+  // 4- and 8-byte cells: the cell first.  Off cells hold zero bytes, so a
+  // nonzero cell is on and its mask word is not read; only a zero cell (off,
+  // or on with a zero value) costs the mask load.
+  uint64_t cv[kLookups];
+  const bool cell_first = vb == 4 || vb == 8;
+#pragma unroll
+  for (uint32_t j = 0; j < kLookups; ++j) {
+    cv[j] = 0;
+    if (cell_first && c[j] < kPendingChild)
+      cv[j] = vb == 4 ? (uint64_t)((const uint32_t BXG_G*)m.cells)[g[j]]
+                      : ((const uint64_t BXG_G*)m.cells)[g[j]];
+  }
+  bool on[kLookups];
+#pragma unroll
+  for (uint32_t j = 0; j < kLookups; ++j) {
+    on[j] = cv[j] != 0;
+    if (!on[j] && c[j] < kPendingChild)
+      on[j] = (m.masks[(uint64_t)c[j] * m.lw + (li[j] >> 6)] >> (li[j] & 63)) & 1ull;
+  }
 #pragma unroll
   for (uint32_t j = 0; j < kLookups; ++j) {
     const uint64_t i = base + (uint64_t)j * blockDim.x;
-    const bool on = (mw[j] >> (li[j] & 63)) & 1;
-    const uint64_t g = (uint64_t)c[j] * m.lsz + li[j];
     if (i >= n) continue;
-    if (found) found[i] = on;
+    if (found) found[i] = on[j];
     if (!vals || !vb) continue;
     if (vb == 4) {
-      ((uint32_t*)vals)[i] = on ? ((const uint32_t BXG_G*)m.cells)[g] : 0u;
+      ((uint32_t*)vals)[i] = (uint32_t)cv[j];
     } else if (vb == 8) {
-      ((uint64_t*)vals)[i] = on ? ((const uint64_t BXG_G*)m.cells)[g] : 0ull;
+      ((uint64_t*)vals)[i] = cv[j];
     } else {
-      for (uint32_t b = 0; b < vb; ++b) vals[i * vb + b] = on ? m.cells[g * vb + b] : 0;
+      for (uint32_t b = 0; b < vb; ++b) vals[i * vb + b] = on[j] ? m.cells[g[j] * vb + b] : 0;
     }
   }
 }
