@@ -45,7 +45,7 @@ def measured_traffic(kernel):
 METRIC = "rays/s + voxel updates/s, batched vs plain per-scan on 1 MI355X; p50/p99 latency"
 
 
-LOOKUP_BYTES = 21
+LOOKUP_BYTES = 28
 
 
 def log(*a):
@@ -53,9 +53,9 @@ def log(*a):
 
 
 def frame_stats(maps):
-    rays = sum(m.stats()["last_rays"] for m in maps)
-    ray_upd = sum(m.stats()["last_ray_updates"] for m in maps)
-    return rays, ray_upd
+    st = [m.stats() for m in maps]
+    return (sum(x["last_rays"] for x in st), sum(x["last_ray_updates"] for x in st),
+            sum(x["last_ray_visits"] for x in st))
 
 
 def run_batched(ctx, bx, scans, dev, steps, warmup, max_range):
@@ -70,7 +70,7 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range):
     ctx.counting(True)
     bx.insert_batch(ctx, descs)
     ctx.counting(False)
-    rays, ray_upd = frame_stats(maps)
+    rays, ray_upd, visits = frame_stats(maps)
     total_points = sum(len(p) for p, _ in scans)
     g = bx.ScanGraph(ctx, len(scans), total_points)
     g.set_scans(descs)
@@ -110,6 +110,7 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range):
         m.close()
     return {
         "total_ms": total_ms, "lat": lat, "rays_traced": rays, "ray_updates": ray_upd,
+        "ray_visits": visits,
         "kernels_ms": kern, "points": total_points,
     }
 
@@ -264,6 +265,51 @@ def run_serving(ctx, bx, scenes, nreq=800, rate=1500.0, nmaps=16, seed=21):
     return out
 
 
+def cpu_threads():
+    """Worker threads for the threaded CPU baseline: the process's CPU share
+    (OMP_NUM_THREADS when set), at most 16."""
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    return max(1, min(n, 16))
+
+
+def host_info():
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_baseline_threads(scans, max_range, budget_s, nthreads):
+    """One map per thread (the maps are independent; the oracle's C calls
+    release the GIL): scans in frame order, nthreads at a time."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(sc):
+        pts, origin = sc
+        m = oracle.ProbMap(0.1)
+        m.insert_point_cloud(pts, origin, max_range)
+        m.close()
+        return len(pts)
+
+    t0 = time.perf_counter()
+    pts_done = maps_done = 0
+    with ThreadPoolExecutor(nthreads) as ex:
+        for i in range(0, len(scans), nthreads):
+            pts_done += sum(ex.map(one, scans[i:i + nthreads]))
+            maps_done += len(scans[i:i + nthreads])
+            if time.perf_counter() - t0 > budget_s:
+                break
+    dt = time.perf_counter() - t0
+    return pts_done / dt, maps_done, pts_done, dt
+
+
 def cpu_baseline(scans, max_range, budget_s):
     import oracle
     t0 = time.perf_counter()
@@ -354,6 +400,11 @@ def main():
     from bonxai_amd import autotune
     ctx_tuning = {k: ctx.get_tuning(k) for k in autotune.CANDIDATES}
     cpu_rate, cpu_maps, cpu_pts, cpu_dt = cpu_baseline(scans, max_range, args.cpu_seconds)
+    nthr = cpu_threads()
+    mt_rate, mt_maps, mt_pts, mt_dt = cpu_baseline_threads(scans, max_range, args.cpu_seconds / 2, nthr)
+    # visits as counted for CPU loops: every ray cell crossed, plus one
+    # endpoint visit per point
+    visits = b["ray_visits"] + npts
     lat = np.array(b["lat"])
     out = {
         "metric": METRIC,
@@ -374,9 +425,11 @@ def main():
                         "one captured graph per frame",
             "maps": len(scans), "points_per_frame": npts, "voxel": 0.1, "max_range": max_range,
             "rays_traced_per_frame": b["rays_traced"], "ray_updates_per_frame": b["ray_updates"],
+            "ray_visits_per_frame": b["ray_visits"], "input_sha256": scenes.digest_scans(scans),
             "tuning": {k: ctx_tuning[k] for k in sorted(ctx_tuning)},
         },
         "voxel_updates_per_s": upd_s,
+        "visits_per_s": visits / (ms_step / 1e3),
         "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "kernels_ms": b["kernels_ms"],
         "roofline": {
@@ -389,6 +442,12 @@ def main():
             "value": cpu_rate, "unit": "rays/s", "cores": 1, "kind": "port",
             "sample": f"the first {cpu_maps} scans of the frame ({cpu_pts} points, {cpu_dt:.1f} s), "
                       "oracle/bx_oracle.c -O2, one thread",
+            "host": host_info(),
+        },
+        "cpu_baseline_threads": {
+            "value": mt_rate, "unit": "rays/s", "cores": nthr, "kind": "port",
+            "sample": f"the first {mt_maps} scans of the frame ({mt_pts} points, {mt_dt:.1f} s), "
+                      f"one map per thread, {nthr} threads",
         },
     }
     if p is not None:
@@ -398,9 +457,9 @@ def main():
                                             "p99": float(np.percentile(p["lat"], 99))}}
         out["batched_over_plain"] = rays_s / plain_rays_s
     if lk is not None:
-        # algorithmic bytes of a lookup: its 12-byte coordinate and 4-byte cell
-        # read, its 4-byte value and 1-byte flag written (a mask word is read
-        # only for a zero cell; root and inner-node reads are shared)
+        # algorithmic bytes of a lookup, the declared model: 12-byte coordinate,
+        # 8-byte mask word and 4-byte cell read, 4-byte value written (root and
+        # inner-node reads are cache-resident and not counted)
         lk_bytes = LOOKUP_BYTES * lk["lookups"]
         lk_gbs = lk_bytes / (lk["lookup_ms"] / 1e3) / 1e9
         out["lookup"] = {**lk, "roofline": {"bound": "hbm", "achieved": lk_gbs, "peak": HBM_PEAK_GBS,

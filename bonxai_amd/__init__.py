@@ -73,7 +73,8 @@ class MapStats(ctypes.Structure):
                 ("root_count", ctypes.c_uint64), ("root_capacity", ctypes.c_uint64),
                 ("scans_applied", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
                 ("last_rays", ctypes.c_uint64), ("last_ray_updates", ctypes.c_uint64),
-                ("update_count", ctypes.c_uint32), ("status", ctypes.c_uint32)]
+                ("update_count", ctypes.c_uint32), ("status", ctypes.c_uint32),
+                ("last_ray_visits", ctypes.c_uint64)]
 
 
 class ScanDesc(ctypes.Structure):

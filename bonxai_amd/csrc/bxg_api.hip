@@ -680,7 +680,7 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
 // reset the per-scan statistics of the maps of a counted run
 int reset_scan_stats(bxg_map* m) {
   bxg_ctx* c = m->ctx;
-  const uint64_t z[2] = {0, 0};
+  const uint64_t z[3] = {0, 0, 0};  // stat_rays, stat_ray_updates, stat_ray_visits
   return upload(c, &m->ctr->stat_rays, z, sizeof(z));
 }
 
@@ -1245,6 +1245,7 @@ int bxg_map_get_stats(bxg_map* m, bxg_map_stats* out) {
   out->device_bytes = m->bytes;
   out->last_rays = k.stat_rays;
   out->last_ray_updates = k.stat_ray_updates;
+  out->last_ray_visits = k.stat_ray_visits;
   out->update_count = (uint32_t)k.cur;
   out->status = k.status;
   return BXG_OK;

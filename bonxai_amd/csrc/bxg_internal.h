@@ -55,7 +55,8 @@ struct alignas(64) MapCounters {
   uint64_t stat_endpoint_updates;
   uint64_t stat_rays;
   uint64_t stat_ray_updates;
-  uint64_t pad[2];
+  uint64_t stat_ray_visits;  // counted runs: cells the scan's rays crossed
+  uint64_t pad;
 };
 
 struct alignas(64) MapDev {

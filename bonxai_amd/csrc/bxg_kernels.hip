@@ -719,7 +719,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
   uint32_t next_leaf = kNone;
   // counted form: per-lane totals, flushed when the lane's map changes
   MapCounters* cmap = nullptr;
-  uint64_t c_upd = 0, c_rays = 0;
+  uint64_t c_upd = 0, c_rays = 0, c_vis = 0;
   for (;;) {
     // hand out rays to the lanes without one (wave-uniform control flow)
     for (;;) {
@@ -758,10 +758,12 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
           if (cmap) {
             add64(&cmap->stat_ray_updates, c_upd);
             add64(&cmap->stat_rays, c_rays);
+            add64(&cmap->stat_ray_visits, c_vis);
           }
           cmap = m.ctr;
-          c_upd = c_rays = 0;
+          c_upd = c_rays = c_vis = 0;
         }
+        if (kMode == 2 && have) c_vis += (uint32_t)w.mx;
       }
       chunk_pos += take;
     }
@@ -784,6 +786,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
   if (kMode == 2 && cmap) {
     add64(&cmap->stat_ray_updates, c_upd);
     add64(&cmap->stat_rays, c_rays);
+    add64(&cmap->stat_ray_visits, c_vis);
   }
 }
 
@@ -890,6 +893,7 @@ __global__ void __launch_bounds__(256) k_scan_extra_rays(const BatchHdr* __restr
     if (kCount) {
       add64(&m.ctr->stat_ray_updates, n);
       add64(&m.ctr->stat_rays, 1);
+      add64(&m.ctr->stat_ray_visits, (uint32_t)w.mx);
     }
   }
 }

@@ -60,6 +60,17 @@ def fnv1a(a):
     return int(lib().bxs_fnv1a(_p(a), a.nbytes))
 
 
+def digest_scans(scans):
+    """SHA-256 (hex) over every scan's float32 points and float64 origin, in
+    order: identical on every machine that builds the same inputs."""
+    import hashlib
+    h = hashlib.sha256()
+    for pts, origin in scans:
+        h.update(np.ascontiguousarray(pts, np.float32).tobytes())
+        h.update(np.ascontiguousarray(origin, np.float64).tobytes())
+    return h.hexdigest()
+
+
 def sincos(x):
     x = np.ascontiguousarray(x, np.float64)
     s = np.empty_like(x)

@@ -80,6 +80,7 @@ typedef struct bxg_map_stats {
   uint64_t last_ray_updates;
   uint32_t update_count;        /* ProbabilisticMap::_update_count (1..3) */
   uint32_t status;              /* device error bits, 0 if none */
+  uint64_t last_ray_visits;     /* counted runs: cells the rays crossed (sum of ray lengths) */
 } bxg_map_stats;
 
 /* one scan of a batched insert */

@@ -97,7 +97,8 @@ def test_lidar_sequence_prefix(ctx):
 def test_lidar_sequence_against_digests(ctx):
     """Sequential LiDAR mapping into one map (64 x 1024 rays a scan, 0.1 m,
     50 m max range, 1 m/scan): the map's order-free digest and the scan's
-    endpoint/ray statistics after every scan equal the oracle's, committed in
+    endpoint/ray statistics (rays, cells updated, cells crossed) after every
+    scan equal the oracle's, committed in
     tests/golden/digests.json (all 200 scans when the full set was generated,
     else its first scans)."""
     import json
@@ -116,6 +117,7 @@ def test_lidar_sequence_against_digests(ctx):
             st = m.stats()
             assert st["last_rays"] == want[k]["stats"]["rays"], f"scan {k} rays"
             assert st["last_ray_updates"] == want[k]["stats"]["ray_updates"], f"scan {k} ray updates"
+            assert st["last_ray_visits"] == want[k]["stats"]["visits"], f"scan {k} ray visits"
     finally:
         ctx.counting(False)
     print(f"{len(want)} scans, {m.stats()['leaf_count']} leaves", flush=True)
@@ -145,6 +147,7 @@ def test_batched_frame_against_digests(ctx):
             assert [str(x) for x in m.digest()] == w["digest"], f"map {w['map']}"
             st = m.stats()
             assert st["last_rays"] == w["stats"]["rays"] and st["last_ray_updates"] == w["stats"]["ray_updates"]
+            assert st["last_ray_visits"] == w["stats"]["visits"], f"map {w['map']} ray visits"
     finally:
         ctx.counting(False)
     print(f"{len(want)} maps checked", flush=True)

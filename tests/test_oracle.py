@@ -267,6 +267,15 @@ def test_input_digests():
         assert scenes.fnv1a(scenes.depth_frame(s)[0]) == d[f"depth_{s}"]
 
 
+# SHA-256 of the batched benchmark frame's inputs (64 scans, points and
+# origins), the value bench.py reports as config.input_sha256
+BATCH_FRAME_SHA256 = "5de68a0120b80bb244587663ae876a9adfdaee3f362d4998564efc90f122cfc6"
+
+
+def test_batch_frame_sha256():
+    assert scenes.digest_scans(scenes.batch_scans()) == BATCH_FRAME_SHA256
+
+
 def test_box_scan_digest():
     d = json.load(open(os.path.join(GOLDEN, "digests.json")))["maps"]["box_scan"]
     pts, origin = scenes.box_scan()
