@@ -27,8 +27,8 @@ TUNING_FILE = os.path.join(HERE, "tuning.json")
 
 CANDIDATES = {
     "ray_block": [64, 128, 256],
-    "ray_blocks_per_cu": [2, 4, 8, 16],
-    "leaf_blocks_per_cu": [4, 8, 16, 32],
+    "ray_blocks_per_cu": [2, 4, 6, 8, 12, 16],
+    "leaf_blocks_per_cu": [4, 6, 8, 12, 16, 32],
     "ray_chunk": [64, 128, 256, 512],
     "ep_table_factor": [2, 4],
     "root_slack": [2, 4],
