@@ -649,10 +649,14 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
   // work-stealing ray kernels: enough blocks to fill every CU, no more
   // than the rays could use (64 rays per wavefront, 4 wavefronts a block)
   const uint32_t cus = (uint32_t)std::max(c->cu_count, 1);
-  // at most one lane per point (the winners are fewer)
+  // leaf pass: at most one lane per point (the winners are fewer); update
+  // pass: up to 8, since a call with fewer rays than lanes walks each ray as
+  // up to 8 segments (on the leaf pass the extra wavefronts cost more in
+  // cursor traffic than the segments gain)
   const uint32_t rb = c->ray_block;
   const uint32_t most = (uint32_t)((grid_points + rb - 1) / rb);
-  const uint32_t gr = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->ray_blocks_per_cu, most));
+  const uint32_t most8 = (uint32_t)std::min<uint64_t>((grid_points * 8 + rb - 1) / rb, 1u << 30);
+  const uint32_t gr = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->ray_blocks_per_cu, most8));
   const uint32_t gl = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->leaf_blocks_per_cu, most));
   sc.chunk_max = c->ray_chunk;
   BXG_TRY(launch(c, "k_scan_classify", k_scan_classify, gp, kBlock, hdr, desc, sc));

@@ -266,7 +266,7 @@ struct RayWalk {
   int32_t sx, sy, sz;
   int32_t dx, dy, dz;
   int32_t ex, ey, ez;
-  int32_t mx, k;
+  int32_t mx, k, kend;
   __device__ __forceinline__ void init(int32_t ox, int32_t oy, int32_t oz, int32_t tx, int32_t ty,
                                        int32_t tz) {
     cx = ox;
@@ -285,8 +285,26 @@ struct RayWalk {
     mx = mx > dz ? mx : dz;
     ex = ey = ez = 0;
     k = 0;
+    kend = mx;
   }
-  __device__ __forceinline__ bool done() const { return k >= mx; }
+  // walk only cells [k0, k1) of the ray: cell k0 and its error terms by the
+  // closed form (n steps along an axis after k cells: n = floor((2k|d| +
+  // mx) / (2mx)), error k|d| - n mx), the same state stepping from 0 reaches
+  __device__ __forceinline__ void seek(int32_t k0, int32_t k1) {
+    const int64_t m2 = 2 * (int64_t)mx;
+    const int64_t nx = (2 * (int64_t)k0 * dx + mx) / m2;
+    const int64_t ny = (2 * (int64_t)k0 * dy + mx) / m2;
+    const int64_t nz = (2 * (int64_t)k0 * dz + mx) / m2;
+    cx += sx * (int32_t)nx;
+    cy += sy * (int32_t)ny;
+    cz += sz * (int32_t)nz;
+    ex = (int32_t)((int64_t)k0 * dx - nx * mx);
+    ey = (int32_t)((int64_t)k0 * dy - ny * mx);
+    ez = (int32_t)((int64_t)k0 * dz - nz * mx);
+    k = k0;
+    kend = k1;
+  }
+  __device__ __forceinline__ bool done() const { return k >= kend; }
   // move to cell k + 1
   __device__ __forceinline__ void advance() {
     ++k;
@@ -668,9 +686,16 @@ __global__ void __launch_bounds__(256) k_scan_scatter(const BatchHdr* __restrict
 
 // Start the ray of winner `idx`; false if it has no cells to walk (origin
 // voxel endpoint, endpoint-only call, or a map whose call failed).
+// With lg > 0, work unit `unit` is segment unit & (2^lg - 1) of winner
+// unit >> lg: segment s of n = 2^lg walks cells [ceil(s mx / n), ceil((s + 1)
+// mx / n)), so segment 0 holds cell 0 and the segments cover the ray once.
+// `first` tells the caller the unit holds the ray's cell 0.
 __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const ScanScratch& sc,
-                                          uint32_t idx, MapView& m, RayWalk& w, RootCache& rc,
-                                          int32_t& cur, bool check_status) {
+                                          uint32_t unit, uint32_t lg, MapView& m, RayWalk& w,
+                                          RootCache& rc, int32_t& cur, bool check_status,
+                                          bool& first) {
+  const uint32_t idx = unit >> lg, seg = unit & ((1u << lg) - 1u);
+  first = seg == 0;
   const uint32_t gi = sc.winners[idx];
   const int4 r = sc.rec[gi];
   const ScanDesc& d = sd[(uint32_t)r.w >> 16];
@@ -680,6 +705,13 @@ __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const
   if (check_status && (map_status(m) & kFatal)) return false;
   cur = map_cur(m);
   w.init(d.ox, d.oy, d.oz, r.x, r.y, r.z);
+  if (lg) {
+    const int64_t n = 1ll << lg;
+    const int32_t k0 = (int32_t)(((int64_t)w.mx * seg + n - 1) >> lg);
+    const int32_t k1 = (int32_t)(((int64_t)w.mx * (seg + 1) + n - 1) >> lg);
+    if (k0 >= k1) return false;
+    w.seek(k0, k1);
+  }
   rc.reset();
   return true;
 }
@@ -687,6 +719,7 @@ __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const
 constexpr uint32_t kChunk = 256;  // default of ScanScratch::chunk_max
 constexpr int32_t kSyncWindow = 16;  // cells a lane may run ahead of its wavefront's slowest ray
 constexpr uint32_t kLockstepRays = 1;  // rays a lane from which batches run in lockstep
+constexpr uint32_t kMaxSegLog = 3;     // at most 8 segments a ray when rays are fewer than lanes
 
 // Work-stealing walk over the winner list.  The list is split into kRegions
 // ranges; a wavefront starts in the range of its XCD (workgroups are
@@ -709,16 +742,23 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
                              uint32_t* cursors) {
   const uint32_t nwin = sc.counters[0];
   const uint32_t lane = lane_id();
+  const uint32_t lanes = gridDim.x * blockDim.x;
+  const bool lockstep = nwin >= kLockstepRays * lanes;
+  // fewer rays than lanes (one scan a call): each ray is walked as 2^lg
+  // segments by different lanes, so the short list still fills the GPU
+  uint32_t lg = 0;
+  while (!lockstep && lg < kMaxSegLog && ((uint64_t)nwin << (lg + 1)) <= lanes) ++lg;
+  const uint32_t nunits = nwin << lg;
   uint32_t chunk_pos = 0, chunk_end = 0;
   // small lists are one region: splitting them only multiplies cursor traffic
   const uint32_t cmax = sc.chunk_max ? sc.chunk_max : kChunk;
-  const uint32_t nreg = nwin >= kRegions * 8 * cmax ? kRegions : 1u;
-  // chunk size: kChunk rays when there is work for every wavefront, down
-  // to one ray a lane on small lists
-  uint32_t chunk = (nwin / (gridDim.x * (blockDim.x / 64u))) & ~63u;
+  const uint32_t nreg = nunits >= kRegions * 8 * cmax ? kRegions : 1u;
+  // chunk size: kChunk units when there is work for every wavefront, down
+  // to one unit a lane on small lists
+  uint32_t chunk = (nunits / (gridDim.x * (blockDim.x / 64u))) & ~63u;
   chunk = chunk < 64u ? 64u : (chunk > cmax ? cmax : chunk);
   uint32_t region = blockIdx.x % nreg, tried = 0;
-  const bool lockstep = nwin >= kLockstepRays * gridDim.x * blockDim.x;
+  bool first = false;
   bool exhausted = false;
   bool have = false;
   MapView m;
@@ -739,8 +779,8 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       const uint64_t need = __ballot(!have);
       if (!refill || need == 0 || exhausted) break;
       if (chunk_pos == chunk_end) {
-        const uint32_t lo = (uint32_t)((uint64_t)nwin * region / nreg);
-        const uint32_t hi = (uint32_t)((uint64_t)nwin * (region + 1) / nreg);
+        const uint32_t lo = (uint32_t)((uint64_t)nunits * region / nreg);
+        const uint32_t hi = (uint32_t)((uint64_t)nunits * (region + 1) / nreg);
         uint32_t base = hi - lo;
         // plain reads first: an exhausted range costs no atomic
         if (lane == 0 && !*gp(&cursors[kRegions + region]) && *gp(&cursors[region]) < hi - lo)
@@ -764,7 +804,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       const uint32_t take = want < avail ? want : avail;
       const uint32_t rank = (uint32_t)__popcll(need & lanes_below());
       if (!have && rank < take) {
-        have = start_ray(sd, sc, chunk_pos + rank, m, w, rc, cur, kMode != 0);
+        have = start_ray(sd, sc, chunk_pos + rank, lg, m, w, rc, cur, kMode != 0, first);
         nupd = 0;
         if (kMode != 0 && have) next_leaf = walk_leaf(m, w, rc);
         if (kMode == 2 && have && m.ctr != cmap) {
@@ -776,7 +816,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
           cmap = m.ctr;
           c_upd = c_rays = c_vis = 0;
         }
-        if (kMode == 2 && have) c_vis += (uint32_t)w.mx;
+        if (kMode == 2 && have) c_vis += (uint32_t)(w.kend - w.k);
       }
       chunk_pos += take;
     }
@@ -803,7 +843,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
         have = false;
         if (kMode == 2) {
           c_upd += nupd;
-          c_rays += 1;
+          c_rays += first ? 1u : 0u;
         }
       }
     }
