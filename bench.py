@@ -188,6 +188,85 @@ def run_lookups(ctx, bx, scenes, reps=20):
     }
 
 
+def run_box_scan(ctx, bx, scenes, reps=10):
+    """Config 1, the reference's own CPU-sized case: one 100k-point box-room
+    scan at 0.05 m, max range 10 m, into a fresh map (pools reserved
+    beforehand, so the timed call allocates no device memory).  Returns the
+    median per-scan time over reps fresh maps and the oracle's time for the
+    same scan on one core."""
+    import oracle
+    pts, origin = scenes.box_scan()
+    d = ctx.to_device(pts)
+    ms = []
+    for _ in range(reps + 1):
+        m = bx.ProbabilisticMap(ctx, 0.05)
+        m.reserve(160_000, 4096)
+        ctx.sync()
+        ctx.event(2)
+        m.insert_point_cloud(d, origin, 10.0, sync=False)
+        ctx.event(3)
+        ms.append(ctx.elapsed_ms(2, 3))
+        if m.sync() != 0:
+            raise RuntimeError("box scan failed")
+        m.close()
+    d.free()
+    t0 = time.perf_counter()
+    om = oracle.ProbMap(0.05)
+    om.insert_point_cloud(pts, origin, 10.0)
+    cpu_s = time.perf_counter() - t0
+    om.close()
+    gpu_ms = float(np.median(ms[1:]))
+    return {"points": len(pts), "ms": gpu_ms, "rays_per_s": len(pts) / (gpu_ms / 1e3),
+            "cpu_ms": cpu_s * 1e3, "cpu_rays_per_s": len(pts) / cpu_s, "cpu_cores": 1}
+
+
+def run_sequence(ctx, bx, scenes, nscans=200, cpu_budget_s=3.0):
+    """Config 3: 200 LiDAR scans (64 x 1024 rays, 0.1 m, 50 m) into one map in
+    order, one insert each (device points, pools reserved from an untimed
+    pass).  The oracle maps the first scans on one core for about
+    cpu_budget_s."""
+    import oracle
+    scans = list(scenes.lidar_sequence(nscans))
+    dev = [ctx.to_device(p) for p, _ in scans]
+    m = bx.ProbabilisticMap(ctx, 0.1)
+    for d, (_, o) in zip(dev, scans):
+        m.insert_point_cloud(d, o, 50.0, sync=False)
+    if m.sync() != 0:
+        raise RuntimeError("sequence sizing pass failed")
+    st = m.stats()
+    m.close()
+    m = bx.ProbabilisticMap(ctx, 0.1)
+    m.reserve(int(st["leaf_count"] * 1.25) + 8192, int(st["root_count"] * 2) + 1024)
+    ctx.sync()
+    ctx.event(2)
+    for d, (_, o) in zip(dev, scans):
+        m.insert_point_cloud(d, o, 50.0, sync=False)
+    ctx.event(3)
+    total_ms = ctx.elapsed_ms(2, 3)
+    if m.sync() != 0:
+        raise RuntimeError("sequence failed")
+    leaves = m.stats()["leaf_count"]
+    m.close()
+    for d in dev:
+        d.free()
+    npts = sum(len(p) for p, _ in scans)
+    om = oracle.ProbMap(0.1)
+    t0 = time.perf_counter()
+    cpu_pts = cpu_scans = 0
+    for pts, o in scans:
+        om.insert_point_cloud(pts, o, 50.0)
+        cpu_pts += len(pts)
+        cpu_scans += 1
+        if time.perf_counter() - t0 > cpu_budget_s:
+            break
+    cpu_s = time.perf_counter() - t0
+    om.close()
+    return {"scans": nscans, "points": npts, "ms": total_ms, "ms_per_scan": total_ms / nscans,
+            "rays_per_s": npts / (total_ms / 1e3), "leaves": leaves,
+            "cpu_rays_per_s": cpu_pts / cpu_s, "cpu_cores": 1,
+            "cpu_sample": f"the first {cpu_scans} scans ({cpu_pts} points, {cpu_s:.1f} s)"}
+
+
 def run_serving(ctx, bx, scenes, nreq=800, rate=1500.0, nmaps=16, seed=21):
     """Serving form: nmaps resident maps at 0.05 m, a Poisson stream of
     requests -- inserts of 640x480 depth frames (max range 5 m), lookup
@@ -337,6 +416,8 @@ def main():
     ap.add_argument("--no-lookups", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the single box scan and the 200-scan LiDAR sequence")
     ap.add_argument("--serve-requests", type=int, default=800)
     ap.add_argument("--serve-rate", type=float, default=1500.0, help="offered requests per second")
     ap.add_argument("--ray-blocks-per-cu", type=int, default=0,
@@ -379,6 +460,12 @@ def main():
     if not args.no_lookups:
         lk = run_lookups(ctx, bx, scenes)
         log(f"lookups: {lk['lookup_ms']:.3f} ms per 8M")
+    c1 = c3 = None
+    if not args.no_configs:
+        c1 = run_box_scan(ctx, bx, scenes)
+        log(f"box scan: {c1['ms']:.3f} ms ({c1['rays_per_s'] / 1e6:.0f} M rays/s), oracle {c1['cpu_ms']:.0f} ms")
+        c3 = run_sequence(ctx, bx, scenes)
+        log(f"lidar sequence: {c3['ms_per_scan']:.3f} ms/scan ({c3['rays_per_s'] / 1e6:.0f} M rays/s)")
     sv = None
     if not args.no_serving:
         sv = run_serving(ctx, bx, scenes, nreq=args.serve_requests, rate=args.serve_rate)
@@ -466,6 +553,9 @@ def main():
                                             "unit": "GB/s", "frac": lk_gbs / HBM_PEAK_GBS,
                                             "traffic": measured_traffic("k_lookup"),
                                             "bytes_per_lookup": LOOKUP_BYTES}}
+    if c1 is not None:
+        out["box_scan"] = c1
+        out["lidar_sequence"] = c3
     if sv is not None:
         out["serving"] = sv
     for d in dev:

@@ -24,6 +24,7 @@ namespace {
 
 constexpr uint32_t kBlock = 256;
 constexpr int kStageSlots = 8;
+constexpr size_t kMaxPendingInserts = 256;  // asynchronous inserts a map holds before it is confirmed
 
 struct PinnedSlot {
   void* host = nullptr;
@@ -116,6 +117,10 @@ struct bxg_map {
   int4* d_extra = nullptr;  // rays due from addHit/MissPoint
   uint32_t extra_cap = 0;
   std::vector<int4> extra;
+  // asynchronous inserts enqueued since the map was last confirmed, and the
+  // device's scans_applied before the first of them (see settle)
+  std::vector<bxg_scan_desc> pending;
+  uint64_t pending_base = 0;
 };
 
 struct bxg_graph {
@@ -780,14 +785,46 @@ int grow_after_overflow(bxg_map* m) {
 
 // ---------------------------------------------------------- bulk ops
 
+// Confirm the asynchronous inserts enqueued on m since it was last confirmed
+// (blocking).  A pool overflow among them is repaired here: the device ran
+// them in order, the first to overflow and every later one on the map wrote
+// no cell, and scans_applied tells how many completed; the pools grow and
+// the rest run again in order, each re-run checked.  Any other call on the
+// map settles first, so it sees every earlier insert applied.  Errors other
+// than an overflow stay on the map for map_status to report.
+int settle(bxg_map* m) {
+  if (m->pending.empty()) return BXG_OK;
+  bxg_ctx* c = m->ctx;
+  std::vector<bxg_scan_desc> todo;
+  todo.swap(m->pending);
+  BXG_TRY(read_counters(m));
+  const uint32_t overflow = kStOutOfPool | kStRootFull;
+  if ((m->h_ctr->status & ~overflow) || !(m->h_ctr->status & overflow)) return BXG_OK;
+  const uint64_t applied = m->h_ctr->scans_applied - m->pending_base;
+  for (uint64_t i = applied; i < todo.size(); ++i) {
+    for (int attempt = 0;; ++attempt) {
+      if (m->h_ctr->status & overflow) {
+        if (attempt > 8) return fail(c, BXG_ERR_OUT_OF_POOL, "could not grow the node pool");
+        BXG_TRY(grow_after_overflow(m));
+      }
+      BXG_TRY(run_batch(c, &todo[i], 1, 0));
+      BXG_TRY(read_counters(m));
+      if (m->h_ctr->status & ~overflow) return BXG_OK;
+      if (!(m->h_ctr->status & overflow)) break;
+    }
+  }
+  return BXG_OK;
+}
+
 int check_coords_args(bxg_map* m, const void* coords, uint64_t n) {
   if (!m) return BXG_ERR_INVALID_ARG;
   if (n && !coords) return fail(m->ctx, BXG_ERR_INVALID_ARG, "null coordinates");
   if (n >= (1ull << 31)) return fail(m->ctx, BXG_ERR_INVALID_ARG, "too many coordinates in one call");
-  return BXG_OK;
+  return settle(m);
 }
 
 int map_failed(bxg_map* m) {
+  BXG_TRY(settle(m));
   poll_counters(m);
   if (!m->ctr_pending && m->h_ctr->status)
     return fail(m->ctx, BXG_ERR_MAP_FAILED, "the map holds a device error; call bxg_map_clear_error");
@@ -1182,6 +1219,7 @@ int bxg_map_destroy(bxg_map* m) {
 int bxg_map_reserve(bxg_map* m, uint64_t leaves, uint64_t roots) {
   if (!m) return BXG_ERR_INVALID_ARG;
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(settle(m));
   if (leaves > m->leaf_cap) BXG_TRY(grow_leaves(m, leaves));
   if (roots * 2 > m->root_cap) BXG_TRY(grow_roots(m, roots));
   return BXG_OK;
@@ -1191,6 +1229,7 @@ int bxg_map_clear(bxg_map* m, int option) {
   if (!m) return BXG_ERR_INVALID_ARG;
   bxg_ctx* c = m->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(settle(m));
   uint32_t nl = 0;
   BXG_TRY(known_leaf_count(m, &nl));
   if (option == BXG_SET_ALL_CELLS_OFF) {
@@ -1221,12 +1260,14 @@ int bxg_map_clear(bxg_map* m, int option) {
 int bxg_map_sync(bxg_map* m) {
   if (!m) return BXG_ERR_INVALID_ARG;
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(settle(m));
   return map_status(m);
 }
 
 int bxg_map_clear_error(bxg_map* m) {
   if (!m) return BXG_ERR_INVALID_ARG;
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(settle(m));
   BXG_TRY(read_counters(m));
   const uint32_t st = m->h_ctr->status;
   BXG_TRY(clear_status(m));
@@ -1239,6 +1280,7 @@ int bxg_map_clear_error(bxg_map* m) {
 int bxg_map_get_stats(bxg_map* m, bxg_map_stats* out) {
   if (!m || !out) return BXG_ERR_INVALID_ARG;
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(settle(m));
   BXG_TRY(read_counters(m));
   const MapCounters& k = *m->h_ctr;
   out->leaf_count = std::min<uint64_t>(k.leaf_count, m->leaf_cap);
@@ -1259,6 +1301,7 @@ int bxg_map_active_cells(bxg_map* m, uint64_t* out) {
   if (!m || !out) return BXG_ERR_INVALID_ARG;
   bxg_ctx* c = m->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(settle(m));
   uint32_t nl = 0;
   BXG_TRY(known_leaf_count(m, &nl));
   BXG_HIP(c, hipMemsetAsync(c->d_small, 0, 8, c->stream));
@@ -1281,6 +1324,7 @@ int bxg_map_digest(bxg_map* m, uint64_t out[4]) {
   if (!m || !out) return BXG_ERR_INVALID_ARG;
   bxg_ctx* c = m->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(settle(m));
   uint32_t nl = 0;
   BXG_TRY(known_leaf_count(m, &nl));
   BXG_HIP(c, hipMemsetAsync(c->d_small, 0, 32, c->stream));
@@ -1415,6 +1459,7 @@ int bxg_export(bxg_map* m, int filter, int32_t* coords, void* vals, uint64_t cap
   if (!m || !count) return BXG_ERR_INVALID_ARG;
   bxg_ctx* c = m->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(settle(m));
   if (filter < 0 || filter > 2) return fail(c, BXG_ERR_INVALID_ARG, "unknown export filter");
   if (filter && !is_prob(m)) return fail(c, BXG_ERR_INVALID_ARG, "filtered export needs a probabilistic map");
   uint32_t nl = 0;
@@ -1514,6 +1559,7 @@ int bxg_export_host(bxg_map* m, int filter, int32_t* coords, void* vals, uint64_
   if (!m || !count) return BXG_ERR_INVALID_ARG;
   bxg_ctx* c = m->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(settle(m));
   const uint32_t vb = m->h.value_bytes;
   void *dc = nullptr, *dv = nullptr;
   if (coords && cap) BXG_TRY(dstage(c, 0, cap * 12, &dc));
@@ -1532,6 +1578,7 @@ int bxg_read_leaves_host(bxg_map* m, const int32_t* origins, uint64_t n, uint64_
   if (!m) return BXG_ERR_INVALID_ARG;
   bxg_ctx* c = m->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(settle(m));
   if (!n) return BXG_OK;
   if (!origins || !masks || !exists || (m->h.value_bytes && !cells) || n >= (1u << 31))
     return fail(c, BXG_ERR_INVALID_ARG, "bad leaf read arguments");
@@ -1586,6 +1633,7 @@ int bxg_leaf_origins_host(bxg_map* m, int32_t* origins, uint64_t cap, uint64_t* 
   if (!m || !count) return BXG_ERR_INVALID_ARG;
   bxg_ctx* c = m->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
+  BXG_TRY(settle(m));
   uint32_t nl = 0;
   BXG_TRY(known_leaf_count(m, &nl));
   *count = nl;
@@ -1603,6 +1651,7 @@ int bxg_leaf_origins_host(bxg_map* m, int32_t* origins, uint64_t cap, uint64_t* 
 int bxg_set_options(bxg_map* m, const bxg_prob_options* o) {
   if (!m || !o) return BXG_ERR_INVALID_ARG;
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(settle(m));
   m->h.opt_miss = o->prob_miss_log;
   m->h.opt_hit = o->prob_hit_log;
   m->h.opt_cmin = o->clamp_min_log;
@@ -1622,6 +1671,7 @@ int bxg_get_options(bxg_map* m, bxg_prob_options* o) {
 int bxg_get_update_count(bxg_map* m, uint32_t* out) {
   if (!m || !out) return BXG_ERR_INVALID_ARG;
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(settle(m));
   BXG_TRY(read_counters(m));
   *out = (uint32_t)m->h_ctr->cur;
   return BXG_OK;
@@ -1629,6 +1679,7 @@ int bxg_get_update_count(bxg_map* m, uint32_t* out) {
 int bxg_set_update_count(bxg_map* m, uint32_t v) {
   if (!m || v < 1 || v > 3) return BXG_ERR_INVALID_ARG;
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  BXG_TRY(settle(m));
   const int32_t cur = (int32_t)v;
   BXG_HIP(m->ctx, hipMemcpyAsync(&m->ctr->cur, &cur, 4, hipMemcpyHostToDevice, m->ctx->stream));
   return ctx_sync(m->ctx);
@@ -1639,14 +1690,35 @@ int bxg_set_update_count(bxg_map* m, uint32_t v) {
 namespace {
 
 // one scan, enqueued (the device-pointer insert entry points)
+// An insert of device points: enqueued and remembered until the map is next
+// confirmed (settle), which re-runs it should a pool overflow have skipped
+// it; the caller keeps the points valid until then.  An insert that also
+// traces rays due from addHit/MissPoint is checked at once (those rays are
+// consumed by the call).
 int insert_async(bxg_map* m, const bxg_scan_desc& s) {
   bxg_ctx* c = m->ctx;
-  BXG_TRY(run_batch(c, &s, 1, 0));
-  // rays due from addHit/MissPoint are traced by this insert
   if (!m->extra.empty()) {
+    BXG_TRY(settle(m));
+    for (int attempt = 0;; ++attempt) {
+      BXG_TRY(run_batch(c, &s, 1, 0));
+      BXG_TRY(read_counters(m));
+      const uint32_t st = m->h_ctr->status;
+      if (!(st & (kStOutOfPool | kStRootFull))) break;
+      if (attempt > 8) return fail(c, BXG_ERR_OUT_OF_POOL, "could not grow the node pool");
+      BXG_TRY(grow_after_overflow(m));
+    }
     BXG_TRY(map_status(m));
     m->extra.clear();
+    return BXG_OK;
   }
+  if (m->pending.empty()) {
+    BXG_TRY(read_counters(m));
+    if (m->h_ctr->status) return map_status(m);
+    m->pending_base = m->h_ctr->scans_applied;
+  }
+  BXG_TRY(run_batch(c, &s, 1, 0));
+  m->pending.push_back(s);
+  if (m->pending.size() >= kMaxPendingInserts) BXG_TRY(settle(m));
   return BXG_OK;
 }
 
@@ -1910,6 +1982,7 @@ int bxg_graph_launch(bxg_graph* g) {
   bxg_ctx* c = g->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   if (!g->loaded) return fail(c, BXG_ERR_INVALID_ARG, "no frame loaded");
+  for (bxg_map* m : g->maps) BXG_TRY(settle(m));
   for (size_t s = 0; s < g->maps.size(); ++s)
     BXG_TRY(ensure_room(g->maps[s], scan_leaf_estimate(g->maps[s], g->npoints[s], g->ranges[s]), 1024));
   BXG_HIP(c, hipGraphLaunch(g->exec, c->stream));

@@ -280,6 +280,39 @@ def test_pool_growth_replay(ctx):
     om.close()
 
 
+def test_async_inserts_replay_after_overflow(ctx):
+    """Asynchronous device inserts whose head-room estimate is far too small
+    (two tiny scans first teach it a small growth): the scans that overflowed
+    the node pool, and every later one, run again in order when the map is
+    next confirmed, and the map equals the oracle's."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    scans = list(scenes.lidar_sequence(4))
+    m = bx.ProbabilisticMap(ctx, 0.1)
+    om = ora.ProbMap(0.1)
+    tiny, o0 = scans[0][0][:300], scans[0][1]
+    for _ in range(2):
+        m.insert_point_cloud(tiny, o0, 50.0)
+        om.insert_point_cloud(tiny, o0, 50.0)
+    cap0 = m.stats()["leaf_capacity"]
+    dev = [ctx.to_device(p) for p, _ in scans]
+    for d, (p, o) in zip(dev, scans):
+        m.insert_point_cloud(d, o, 50.0, sync=False)
+        om.insert_point_cloud(p, o, 50.0)
+    assert m.sync() == 0
+    st = m.stats()
+    print(f"leaf capacity {cap0} -> {st['leaf_capacity']}, {st['leaf_count']} leaves", flush=True)
+    assert st["leaf_count"] > cap0  # the async scans could not fit the pool they were given
+    assert_same_cells(_gpu_cells(m), om.cells(), "async inserts after overflow")
+    assert m.update_count == om.update_count
+    assert m.digest() == om.digest()
+    for d in dev:
+        d.free()
+    m.close()
+    om.close()
+
+
 def test_occupied_free_and_classify(ctx):
     import bonxai_amd as bx
     from bonxai_amd import scenes
