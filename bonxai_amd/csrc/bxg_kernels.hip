@@ -345,18 +345,99 @@ __device__ __forceinline__ uint32_t collect_run(const MapView& m, RayWalk& w, ui
   return n;
 }
 
-// One leaf of a ray, allocation pass: create the root and the leaf of the
-// current cell, then move past the ray's cells in that leaf (kRun at a time).
-__device__ __forceinline__ void leaves_step(const MapView& m, RayWalk& w, RootCache& rc) {
-  const int32_t lmask = ~((1 << m.lb) - 1);
-  const int32_t x = w.cx, y = w.cy, z = w.cz;
-  const int32_t slot = root_slot<true>(m, rc, x, y, z);
-  if (slot >= 0)
-    leaf_request(m, (uint32_t)slot, inner_index(x, y, z, m.lb, m.ib), x & lmask, y & lmask, z & lmask);
-  uint32_t li[kRun];
-  while (collect_run(m, w, li) == (uint32_t)kRun && !w.done() && (w.cx & lmask) == (x & lmask) &&
-         (w.cy & lmask) == (y & lmask) && (w.cz & lmask) == (z & lmask)) {
+// The leaves a ray's cells [k, kend) lie in, one leaf per step, without
+// visiting the cells.  Along an axis with |d| > 0 the ray takes its n-th step
+// at cell ceil((2n - 1) m / (2|d|)) (the inverse of the closed form), so the
+// cell at which it enters the leaf N axis steps away is that value for N;
+// the next boundary is L steps further, which adds floor(L m / |d|) and a
+// remainder to the quotient, so after the set-up (a few divisions a ray)
+// each leaf costs additions and compares only.  Axes crossing at the same
+// cell move the ray diagonally, into one leaf.  Same leaves, in the same
+// order, as the cell walk.
+struct LeafWalk {
+  int32_t lx, ly, lz;  // origin of the current leaf
+  int32_t stx, sty, stz;  // +-L per leaf along each axis
+  int32_t kx, ky, kz;  // cell at which the ray enters the next leaf along each axis
+  uint32_t qx, qy, qz, rx, ry, rz;  // kx = qx + (rx > 0): (2N - 1) m = qx * 2|dx| + rx
+  uint32_t Qx, Qy, Qz, Rx, Ry, Rz;  // per further leaf: floor(L m / |d|), 2 (L m mod |d|)
+  uint32_t bx, by, bz;  // 2|d|
+  int32_t k, kend;
+  static constexpr int32_t kNever = 0x7fffffff;
+  __device__ __forceinline__ static void axis(int32_t c, int32_t s, int32_t d, int32_t n, int32_t m,
+                                              uint32_t L, int32_t& lc, int32_t& st, int32_t& kc,
+                                              uint32_t& q, uint32_t& r, uint32_t& Q, uint32_t& R,
+                                              uint32_t& b) {
+    const int32_t lm = (int32_t)L - 1;
+    lc = c & ~lm;
+    st = s * (int32_t)L;
+    b = 2u * (uint32_t)(d > 0 ? d : 1);
+    if (d == 0) {
+      kc = kNever;
+      q = r = Q = R = 0;
+      return;
+    }
+    // axis steps from the origin to the first cell of the next leaf
+    const int64_t N = (int64_t)n + (s > 0 ? (int64_t)L - (c & lm) : (int64_t)(c & lm) + 1);
+    const uint64_t A = (uint64_t)(2 * N - 1) * (uint64_t)m;
+    const uint64_t q64 = A / b;
+    const uint64_t LM = (uint64_t)L * (uint64_t)m;
+    const uint64_t Q64 = LM / (uint64_t)d;
+    // quotients past 2^30 are beyond any ray (kend < 2^30): such an axis
+    // never crosses again, and the sums below cannot overflow
+    q = (uint32_t)(q64 < (1ull << 30) ? q64 : (1ull << 30));
+    r = (uint32_t)(A - q64 * b);
+    Q = (uint32_t)(Q64 < (1ull << 30) ? Q64 : (1ull << 30));
+    R = 2u * (uint32_t)(LM - Q64 * (uint64_t)d);
+    kc = (int32_t)(q + (r > 0 ? 1u : 0u));
   }
+  // the leaf of cell w.k, walking cells [w.k, w.kend) of w's ray
+  __device__ __forceinline__ void init(const RayWalk& w, uint32_t lb) {
+    const uint32_t L = 1u << lb;
+    k = w.k;
+    kend = w.kend;
+    const int64_t m2 = 2 * (int64_t)w.mx;
+    // axis steps already taken at cell k (none at k = 0)
+    const int32_t nx = k ? (int32_t)((2 * (int64_t)k * w.dx + w.mx) / m2) : 0;
+    const int32_t ny = k ? (int32_t)((2 * (int64_t)k * w.dy + w.mx) / m2) : 0;
+    const int32_t nz = k ? (int32_t)((2 * (int64_t)k * w.dz + w.mx) / m2) : 0;
+    axis(w.cx, w.sx, w.dx, nx, w.mx, L, lx, stx, kx, qx, rx, Qx, Rx, bx);
+    axis(w.cy, w.sy, w.dy, ny, w.mx, L, ly, sty, ky, qy, ry, Qy, Ry, by);
+    axis(w.cz, w.sz, w.dz, nz, w.mx, L, lz, stz, kz, qz, rz, Qz, Rz, bz);
+  }
+  __device__ __forceinline__ bool done() const { return k >= kend; }
+  __device__ __forceinline__ static void cross(int32_t& lc, int32_t st, int32_t& kc, uint32_t& q,
+                                               uint32_t& r, uint32_t Q, uint32_t R, uint32_t b) {
+    lc += st;
+    q += Q;
+    r += R;
+    if (r >= b) {
+      ++q;
+      r -= b;
+    }
+    kc = q < (1u << 30) ? (int32_t)(q + (r > 0 ? 1u : 0u)) : kNever;
+  }
+  // move to the next leaf (or past kend)
+  __device__ __forceinline__ void next() {
+    int32_t kn = kx < ky ? kx : ky;
+    kn = kn < kz ? kn : kz;
+    if (kn >= kend) {
+      k = kend;
+      return;
+    }
+    if (kx == kn) cross(lx, stx, kx, qx, rx, Qx, Rx, bx);
+    if (ky == kn) cross(ly, sty, ky, qy, ry, Qy, Ry, by);
+    if (kz == kn) cross(lz, stz, kz, qz, rz, Qz, Rz, bz);
+    k = kn;
+  }
+};
+
+// One leaf of a ray, allocation pass: create the root and the leaf, then
+// move to the ray's next leaf.
+__device__ __forceinline__ void leaves_step(const MapView& m, LeafWalk& lw, RootCache& rc) {
+  const int32_t slot = root_slot<true>(m, rc, lw.lx, lw.ly, lw.lz);
+  if (slot >= 0)
+    leaf_request(m, (uint32_t)slot, inner_index(lw.lx, lw.ly, lw.lz, m.lb, m.ib), lw.lx, lw.ly, lw.lz);
+  lw.next();
 }
 
 // One leaf of a ray, update pass: the cells of the ray in the current leaf
@@ -763,6 +844,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
   bool have = false;
   MapView m;
   RayWalk w;
+  LeafWalk lw;  // the leaf pass walks leaves, not cells
   RootCache rc;
   int32_t cur = 0;
   uint32_t nupd = 0;
@@ -806,6 +888,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       if (!have && rank < take) {
         have = start_ray(sd, sc, chunk_pos + rank, lg, m, w, rc, cur, kMode != 0, first);
         nupd = 0;
+        if (kMode == 0 && have) lw.init(w, m.lb);
         if (kMode != 0 && have) next_leaf = walk_leaf(m, w, rc);
         if (kMode == 2 && have && m.ctr != cmap) {
           if (cmap) {
@@ -825,7 +908,8 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
     // slowest ray waits a step, so the batch stays at about one distance
     // from the sensor (runs end at leaf borders, so rays drift apart
     // otherwise; measured: the update pass about 8 % faster)
-    int32_t kmin = have ? w.k : 0x7fffffff;
+    const int32_t kk = kMode == 0 ? lw.k : w.k;
+    int32_t kmin = have ? kk : 0x7fffffff;
     if (lockstep) {
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
@@ -833,13 +917,13 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
         kmin = t < kmin ? t : kmin;
       }
     }
-    if (have && (!lockstep || w.k < kmin + kSyncWindow)) {
+    if (have && (!lockstep || kk < kmin + kSyncWindow)) {
       if (kMode == 0) {
-        leaves_step(m, w, rc);
+        leaves_step(m, lw, rc);
       } else {
         nupd += apply_step<kMode == 2>(m, w, rc, cur, next_leaf);
       }
-      if (w.done()) {
+      if (kMode == 0 ? lw.done() : w.done()) {
         have = false;
         if (kMode == 2) {
           c_upd += nupd;
@@ -877,7 +961,9 @@ __global__ void __launch_bounds__(256) k_scan_extra_leaves(const BatchHdr* __res
     RootCache rc;
     rc.reset();
     w.init(d.ox, d.oy, d.oz, e.x, e.y, e.z);
-    while (!w.done()) leaves_step(m, w, rc);
+    LeafWalk lw;
+    lw.init(w, m.lb);
+    while (!lw.done()) leaves_step(m, lw, rc);
   }
 }
 
