@@ -280,6 +280,34 @@ def test_pool_growth_replay(ctx):
     om.close()
 
 
+@pytest.mark.parametrize("bits", [(1, 2), (2, 4), (3, 1)])
+def test_other_leaf_sizes_same_cells(ctx, bits):
+    """A scan's cells and values do not depend on how the grid is cut into
+    leaves: maps with 4^3, 16^3 and 2^3 leaves (the ray and leaf walks at
+    those sizes) equal the oracle's 8^3-leaf map, for a box scan and a LiDAR
+    scan."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+
+    class Map(bx.ProbabilisticMap):
+        def __init__(self, c, res, ib, lb):
+            bx.VoxelGrid.__init__(self, c, res, ib, lb, np.uint32, _flags=bx.MAP_PROBABILISTIC)
+
+    ora = _oracle()
+    pts, origin = scenes.box_scan(seed=9, n=30_000)
+    lidar, lo = next(iter(scenes.lidar_sequence(1)))
+    for res, (p, o, r) in ((0.05, (pts, origin, 10.0)), (0.1, (lidar, lo, 50.0))):
+        m = Map(ctx, res, *bits)
+        om = ora.ProbMap(res)
+        for _ in range(2):
+            m.insert_point_cloud(p, o, r)
+            om.insert_point_cloud(p, o, r)
+        assert_same_cells(_gpu_cells(m), om.cells(), f"bits {bits} res {res}")
+        assert m.digest() == om.digest()
+        m.close()
+        om.close()
+
+
 def test_async_inserts_replay_after_overflow(ctx):
     """Asynchronous device inserts whose head-room estimate is far too small
     (two tiny scans first teach it a small growth): the scans that overflowed
