@@ -661,7 +661,13 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
   const uint32_t rb = c->ray_block;
   const uint32_t most = (uint32_t)((grid_points + rb - 1) / rb);
   const uint32_t most8 = (uint32_t)std::min<uint64_t>((grid_points * 8 + rb - 1) / rb, 1u << 30);
-  const uint32_t gr = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->ray_blocks_per_cu, most8));
+  // the tuned grid serves large calls (batched frames, lockstep batches); a
+  // call with fewer points than that grid has lanes walks ray segments on a
+  // grid of 4 workgroups per CU, where more would only queue on one cursor
+  const uint32_t gr_full = std::max<uint32_t>(1, cus * c->ray_blocks_per_cu);
+  const uint32_t gr = grid_points >= (uint64_t)gr_full * rb
+                          ? gr_full
+                          : std::max<uint32_t>(1, std::min<uint32_t>(cus * 4u, most8));
   const uint32_t gl = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->leaf_blocks_per_cu, most));
   sc.chunk_max = c->ray_chunk;
   BXG_TRY(launch(c, "k_scan_classify", k_scan_classify, gp, kBlock, hdr, desc, sc));
