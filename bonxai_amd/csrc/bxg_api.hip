@@ -468,8 +468,7 @@ int ensure_scan_scratch(bxg_ctx* c, uint64_t points, uint32_t nscans) {
     dfree(c->sc.tpos);
     dfree(c->sc.winners);
     dfree(c->sc.wkey);
-    dfree(c->sc.ep_keys);
-    dfree(c->sc.ep_first);
+    dfree(c->sc.ep);
     uint32_t* hist = c->sc.hist;
     uint32_t* offs = c->sc.offs;
     c->sc = ScanScratch{};
@@ -494,13 +493,10 @@ int ensure_scan_scratch(bxg_ctx* c, uint64_t points, uint32_t nscans) {
       c->sc.offs = (uint32_t*)p;
       BXG_TRY(fill32(c, c->sc.hist, kMaxBuckets, 0));
     }
-    BXG_TRY(dmalloc(c, &p, tcap * 8));
-    c->sc.ep_keys = (uint64_t*)p;
-    BXG_TRY(dmalloc(c, &p, tcap * 4));
-    c->sc.ep_first = (uint32_t*)p;
+    BXG_TRY(dmalloc(c, &p, tcap * sizeof(EpSlot)));
+    c->sc.ep = (EpSlot*)p;
     c->sc.ep_mask = (uint32_t)(tcap - 1);
-    BXG_TRY(fill64(c, c->sc.ep_keys, tcap, kEmptyKey));
-    BXG_TRY(fill32(c, c->sc.ep_first, tcap, kNone));
+    BXG_TRY(fill64(c, (uint64_t*)c->sc.ep, tcap * 2, ~0ull));  // empty: all ones
     c->scan_cap = cap;
   }
   c->sc.counters = c->d_counters;
@@ -934,8 +930,7 @@ int bxg_ctx_destroy(bxg_ctx* c) {
   dfree(c->sc.wkey);
   dfree(c->sc.hist);
   dfree(c->sc.offs);
-  dfree(c->sc.ep_keys);
-  dfree(c->sc.ep_first);
+  dfree(c->sc.ep);
   dfree(c->d_counters);
   dfree(c->d_desc);
   dfree(c->d_hdr);
@@ -1907,15 +1902,13 @@ int bxg_graph_create(bxg_ctx* c, uint32_t max_scans, uint64_t max_points, bxg_gr
   if (rc == BXG_OK) rc = dmalloc(c, &p, (uint64_t)kMaxBuckets * 4), g->sc.hist = (uint32_t*)p;
   if (rc == BXG_OK) rc = dmalloc(c, &p, (uint64_t)kMaxBuckets * 4), g->sc.offs = (uint32_t*)p;
   if (rc == BXG_OK) rc = fill32(c, g->sc.hist, kMaxBuckets, 0);
-  if (rc == BXG_OK) rc = dmalloc(c, &p, tcap * 8), g->sc.ep_keys = (uint64_t*)p;
-  if (rc == BXG_OK) rc = dmalloc(c, &p, tcap * 4), g->sc.ep_first = (uint32_t*)p;
+  if (rc == BXG_OK) rc = dmalloc(c, &p, tcap * sizeof(EpSlot)), g->sc.ep = (EpSlot*)p;
   if (rc == BXG_OK) rc = dmalloc(c, &p, kCounterWords * 4), g->d_counters = (uint32_t*)p;
   if (rc == BXG_OK) rc = dmalloc(c, &p, (uint64_t)max_scans * sizeof(ScanDesc)), g->d_desc = (ScanDesc*)p;
   if (rc == BXG_OK) rc = dmalloc(c, &p, sizeof(BatchHdr)), g->d_hdr = (BatchHdr*)p;
   g->sc.ep_mask = (uint32_t)(tcap - 1);
   g->sc.counters = g->d_counters;
-  if (rc == BXG_OK) rc = fill64(c, g->sc.ep_keys, tcap, kEmptyKey);
-  if (rc == BXG_OK) rc = fill32(c, g->sc.ep_first, tcap, kNone);
+  if (rc == BXG_OK) rc = fill64(c, (uint64_t*)g->sc.ep, tcap * 2, ~0ull);  // empty: all ones
   if (rc == BXG_OK) rc = fill32(c, g->d_counters, kCounterWords, 0);
   if (rc == BXG_OK) {
     BatchHdr hdr{0, 0, 0};
@@ -2009,8 +2002,7 @@ int bxg_graph_destroy(bxg_graph* g) {
   dfree(g->sc.wkey);
   dfree(g->sc.hist);
   dfree(g->sc.offs);
-  dfree(g->sc.ep_keys);
-  dfree(g->sc.ep_first);
+  dfree(g->sc.ep);
   dfree(g->d_counters);
   dfree(g->d_desc);
   dfree(g->d_hdr);

@@ -125,11 +125,21 @@ constexpr uint32_t kCounterWords = kTileWords + kTiles;
 static_assert(1 + 4 * kRegions <= kTileWords, "counter words");
 
 // scratch of one integration call, owned by the context
+// One slot of the per-call endpoint table: key (scan tag, root slot, inner
+// and leaf index of a voxel) and the lowest index of the points hitting it,
+// side by side so a point's claim touches one cache line.  An empty slot is
+// all ones (kEmptyKey, kNone).
+struct alignas(16) EpSlot {
+  uint64_t key;
+  uint32_t first;
+  uint32_t pad;
+};
+static_assert(kEmptyKey == ~0ull && kNone == 0xFFFFFFFFu, "an empty EpSlot is all ones");
+
 struct ScanScratch {
   int4* rec;            // per flattened point: endpoint xyz, flags | scan << 16
   uint32_t* tpos;       // per point: endpoint-table slot of its endpoint
-  uint64_t* ep_keys;    // endpoint table keys
-  uint32_t* ep_first;   // endpoint table: min point index
+  EpSlot* ep;           // endpoint table
   uint32_t ep_mask;     // capacity - 1
   uint32_t chunk_max;   // most rays a wavefront takes per cursor atomic (64..1024)
   uint32_t* winners;    // flattened point indices of per-voxel winners, bucketed

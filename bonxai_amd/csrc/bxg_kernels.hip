@@ -599,13 +599,13 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
     uint32_t h = (uint32_t)mix64(key) & sc.ep_mask;
     uint32_t tp = kNone;
     for (uint32_t p = 0; p <= sc.ep_mask; ++p) {
-      uint64_t k = gp(sc.ep_keys)[h];
+      uint64_t k = gp(&sc.ep[h])->key;
       if (k == kEmptyKey) {
-        k = cas64(&sc.ep_keys[h], kEmptyKey, key);
+        k = cas64(&sc.ep[h].key, kEmptyKey, key);
         if (k == kEmptyKey) k = key;
       }
       if (k == key) {
-        min32(&sc.ep_first[h], (uint32_t)gi);
+        min32(&sc.ep[h].first, (uint32_t)gi);
         tp = h;
         break;
       }
@@ -671,7 +671,7 @@ __global__ void __launch_bounds__(256) k_scan_winners(const BatchHdr* __restrict
   uint32_t key = 0;
   if (gi < hdr->total) {
     const int4 r = sc.rec[gi];
-    if ((r.w & kRecMarked) && sc.ep_first[sc.tpos[gi]] == (uint32_t)gi) {
+    if ((r.w & kRecMarked) && gp(&sc.ep[sc.tpos[gi]])->first == (uint32_t)gi) {
       win = true;
       const uint32_t s = (uint32_t)r.w >> 16;
       const ScanDesc& d = sd[s];
@@ -984,8 +984,9 @@ __global__ void __launch_bounds__(256) k_scan_endpoints(const ScanDesc* __restri
       const uint32_t gi = sc.winners[idx];
       const int4 r = sc.rec[gi];
       const uint32_t t = sc.tpos[gi];
-      sc.ep_keys[t] = kEmptyKey;
-      sc.ep_first[t] = kNone;
+      EpSlot BXG_G* e = gp(&sc.ep[t]);
+      e->key = kEmptyKey;
+      e->first = kNone;
       const ScanDesc& d = sd[(uint32_t)r.w >> 16];
       const MapView m = load_view(d.map);
       const uint32_t leaf = (map_status(m) & kFatal) ? kNone : find_leaf(m, r.x, r.y, r.z);
