@@ -838,14 +838,16 @@ int cell_op(bxg_map* m, const int32_t* coords, const void* vals, uint64_t n, uin
   bxg_ctx* c = m->ctx;
   BXG_TRY(check_coords_args(m, coords, n));
   if (!n) return BXG_OK;
-  if (op != kOpSetOff) {
+  if (op != kOpSetOff)
     BXG_TRY(ensure_room(m, std::min<uint64_t>(n, 1u << 22), std::min<uint64_t>(n, 1u << 16)));
-    BXG_TRY(launch(c, "k_touch", k_touch, nblocks(n), kBlock, (const MapDev*)m->d, coords, n));
-  }
   BXG_TRY(ensure_op_scratch(c, n));
   const OpScratch os = op_scratch(c);
-  BXG_TRY(launch(c, "k_op_resolve", k_op_resolve, nblocks(n), kBlock, (const MapDev*)m->d, coords,
-                 n, os));
+  if (op != kOpSetOff)
+    BXG_TRY(launch(c, "k_op_resolve", k_op_resolve<true>, nblocks(n), kBlock, (const MapDev*)m->d,
+                   coords, n, os));
+  else
+    BXG_TRY(launch(c, "k_op_resolve", k_op_resolve<false>, nblocks(n), kBlock, (const MapDev*)m->d,
+                   coords, n, os));
   BXG_TRY(launch(c, "k_op_apply", k_op_apply, nblocks(n), kBlock, (const MapDev*)m->d, coords,
                  (const uint8_t*)vals, n, op, was_on, os));
   BXG_TRY(launch(c, "k_op_release", k_op_release, nblocks(n), kBlock, n, os));

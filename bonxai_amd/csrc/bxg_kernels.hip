@@ -1094,7 +1094,13 @@ __global__ void __launch_bounds__(256) k_touch(const MapDev* __restrict__ mp,
   leaf_request(m, (uint32_t)slot, inner_index(x, y, z, m.lb, m.ib), x & lmask, y & lmask, z & lmask);
 }
 
-// first and last input index of every distinct cell, and its prior mask bit
+// First and last input index of every distinct cell, and its prior mask
+// bit, in one pass with the creation of missing roots and leaves (kCreate:
+// setValue / setCellOn; setCellOff creates nothing and skips absent cells).
+// A cell's table key is its root slot, inner index and leaf index, known
+// before its leaf is, so a lane never waits for another lane's leaf: a leaf
+// still being created (or created in this call) has all mask bits off.
+template <bool kCreate>
 __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ mp,
                                                     const int32_t* __restrict__ coords,
                                                     uint64_t n, OpScratch os) {
@@ -1102,15 +1108,29 @@ __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ m
   if (i >= n) return;
   const MapView m = load_view(mp);
   const int32_t x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
-  const uint32_t leaf = find_leaf(m, x, y, z);
-  if (leaf == kNone) {
-    os.tpos[i] = kNone;
-    os.prebit[i] = 0;
+  os.tpos[i] = kNone;
+  os.prebit[i] = 0;
+  uint64_t rkey;
+  if (!pack_root(x, y, z, m.log2n, &rkey)) {
+    if (kCreate) or32(&m.ctr->status, kStCoordRange);
     return;
   }
+  const int32_t slot = kCreate ? root_insert(m, rkey) : root_find(m, rkey);
+  if (slot < 0) return;
+  const uint32_t ci = inner_index(x, y, z, m.lb, m.ib);
   const uint32_t li = leaf_index(x, y, z, m.lb);
-  const uint64_t key = (uint64_t)leaf * m.lsz + li;
-  os.prebit[i] = (uint8_t)((m.masks[(uint64_t)leaf * m.lw + (li >> 6)] >> (li & 63)) & 1);
+  uint32_t leaf;
+  if (kCreate) {
+    const int32_t lmask = ~((1 << m.lb) - 1);
+    leaf = leaf_request(m, (uint32_t)slot, ci, x & lmask, y & lmask, z & lmask);
+    if (leaf == kEmptyChild) return;  // pool exhausted: the call fails
+  } else {
+    leaf = m.children[(uint64_t)slot * m.isz + ci];
+    if (leaf >= kPendingChild) return;
+  }
+  if (leaf < kPendingChild)
+    os.prebit[i] = (uint8_t)((m.masks[(uint64_t)leaf * m.lw + (li >> 6)] >> (li & 63)) & 1);
+  const uint64_t key = ((uint64_t)(uint32_t)slot << 24) | ((uint64_t)ci << 12) | (uint64_t)li;
   uint32_t h = (uint32_t)mix64(key) & os.mask;
   for (uint32_t p = 0; p <= os.mask; ++p) {
     uint64_t k = gp(os.keys)[h];
@@ -1126,7 +1146,6 @@ __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ m
     }
     h = (h + 1) & os.mask;
   }
-  os.tpos[i] = kNone;
 }
 
 enum : uint32_t { kOpSetValue = 0, kOpSetOn = 1, kOpSetOff = 2 };
