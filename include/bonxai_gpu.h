@@ -217,11 +217,12 @@ int bxg_set_update_count(bxg_map* map, uint32_t value);
 /* ProbabilisticMap::insertPointCloud(points, origin, max_range),
  * probabilistic_map.hpp:91-93,141-160.  points_dev: n xyz triples (float,
  * or double with BXG_POINTS_F64); origin already widened to double.
- * Enqueues only.  If the map runs out of nodes the scan changes no cell and
- * does not advance update_count, later scans of the map are skipped the
- * same way, and bxg_map_sync reports BXG_ERR_OUT_OF_POOL; bxg_map_clear_error
- * grows the pools, after which the skipped scans can be submitted again
- * (bxg_map_get_stats().scans_applied counts the completed ones). */
+ * Enqueues only.  The map keeps the call until it is next confirmed: by
+ * bxg_map_sync, by any other call on the map, or after 256 such inserts.  If
+ * the map ran out of nodes, the scan that did changed no cell, later scans
+ * of the map were skipped the same way, and the confirmation grows the pools
+ * and runs them again in order.  Keep points_dev valid until then.  Other
+ * device errors (a coordinate out of range) are reported by bxg_map_sync. */
 int bxg_insert_point_cloud(bxg_map* map, const void* points_dev, uint64_t n, uint32_t flags,
                            const double origin[3], double max_range);
 /* same, host points; blocks, and grows the pools and re-runs the scan if it

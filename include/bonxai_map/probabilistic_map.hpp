@@ -210,7 +210,9 @@ class ProbabilisticMap {
 
   /// The same for xyz triples already in GPU memory (float, or double with
   /// f64); enqueued on the map's stream without waiting when no host page is
-  /// cached.
+  /// cached.  The map keeps the call until its next confirmation (any other
+  /// call on the map, or bxg_map_sync), which re-runs it after growing the
+  /// pools should they have overflowed: keep points_dev valid until then.
   void insertPointCloudDevice(const void* points_dev, size_t n, bool f64, const Vector3D& origin,
                               double max_range) {
     const double o[3] = {origin.x(), origin.y(), origin.z()};
