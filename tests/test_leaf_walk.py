@@ -111,3 +111,28 @@ def test_long_rays_against_closed_form():
                 assert tuple(v & ~(L - 1) for v in _cell(o, d, s, m, k - 1)) != leaf
         if len(lv) < 400:
             assert tuple(v & ~(L - 1) for v in _cell(o, d, s, m, k1 - 1)) == lv[-1][0]
+
+
+def test_segment_start_equals_stepping():
+    """RayWalk::seek (a segment of a ray starting mid-way, used when a call has
+    fewer rays than lanes): position n = floor((2k|d| + m) / (2m)) steps along
+    each axis and error k|d| - n m, the state the error walk of
+    probabilistic_map.hpp:162-203 reaches after k steps."""
+    rng = random.Random(1)
+    for _ in range(3000):
+        o = [rng.randint(-50, 50) for _ in range(3)]
+        e = [o[i] + rng.randint(-60, 60) for i in range(3)]
+        d = [e[i] - o[i] for i in range(3)]
+        s = [(-1 if v < 0 else 1) for v in d]
+        d = [abs(v) for v in d]
+        m = max(d)
+        c, er = list(o), [0, 0, 0]
+        for k in range(m):
+            n = [(2 * k * d[i] + m) // (2 * m) for i in range(3)]
+            assert [o[i] + s[i] * n[i] for i in range(3)] == c
+            assert [k * d[i] - n[i] * m for i in range(3)] == er
+            for i in range(3):
+                er[i] += d[i]
+                if 2 * er[i] >= m:
+                    c[i] += s[i]
+                    er[i] -= m
