@@ -431,13 +431,37 @@ struct LeafWalk {
   }
 };
 
-// One leaf of a ray, allocation pass: create the root and the leaf, then
-// move to the ray's next leaf.
+// Allocation pass, one step of a ray: the root of its current leaf (created
+// if missing) and up to kLeafRun of its next leaves in that root; their
+// inner-grid entries are loaded together (one memory round trip, not one a
+// leaf), and only those still empty take the creating path.
+constexpr int kLeafRun = 2;
 __device__ __forceinline__ void leaves_step(const MapView& m, LeafWalk& lw, RootCache& rc) {
   const int32_t slot = root_slot<true>(m, rc, lw.lx, lw.ly, lw.lz);
-  if (slot >= 0)
-    leaf_request(m, (uint32_t)slot, inner_index(lw.lx, lw.ly, lw.lz, m.lb, m.ib), lw.lx, lw.ly, lw.lz);
-  lw.next();
+  const int32_t rmask = ~((1 << m.log2n) - 1);
+  int32_t lx[kLeafRun], ly[kLeafRun], lz[kLeafRun];
+  int n = 0;
+#pragma unroll
+  for (int j = 0; j < kLeafRun; ++j) {
+    if (j == 0 || (n == j && !lw.done() && (lw.lx & rmask) == rc.rx && (lw.ly & rmask) == rc.ry &&
+                   (lw.lz & rmask) == rc.rz)) {
+      lx[j] = lw.lx;
+      ly[j] = lw.ly;
+      lz[j] = lw.lz;
+      n = j + 1;
+      lw.next();
+    }
+  }
+  if (slot < 0) return;  // root directory full: the call fails
+  uint32_t v[kLeafRun];
+#pragma unroll
+  for (int j = 0; j < kLeafRun; ++j)
+    v[j] = j < n ? m.children[(uint64_t)slot * m.isz + inner_index(lx[j], ly[j], lz[j], m.lb, m.ib)]
+                 : 0u;
+#pragma unroll
+  for (int j = 0; j < kLeafRun; ++j)
+    if (j < n && v[j] == kEmptyChild)
+      leaf_request(m, (uint32_t)slot, inner_index(lx[j], ly[j], lz[j], m.lb, m.ib), lx[j], ly[j], lz[j]);
 }
 
 // One leaf of a ray, update pass: the cells of the ray in the current leaf
