@@ -7,6 +7,7 @@
 // device header (MapDev) that every kernel reads, so pools can grow between
 // calls without re-capturing graphs.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -136,6 +137,10 @@ struct bxg_graph {
   std::vector<bxg_map*> maps;  // maps of the loaded frame
   std::vector<uint64_t> npoints;
   std::vector<double> ranges;
+  // the loaded frame's scans (with their own copy of each transform), which
+  // each launch leaves pending on its map until the map is confirmed
+  std::vector<bxg_scan_desc> scans;
+  std::vector<std::array<float, 16>> tfs;
   bool loaded = false;
 };
 
@@ -786,6 +791,19 @@ int grow_after_overflow(bxg_map* m) {
 }
 
 // ---------------------------------------------------------- bulk ops
+
+// Start a pending window on m if none is open: the device's scans_applied
+// before the scans about to be enqueued, from the counter mirror when no
+// snapshot is in flight (else read, blocking).  A map holding a device error
+// reports it instead.
+int pending_begin(bxg_map* m) {
+  if (!m->pending.empty()) return BXG_OK;
+  poll_counters(m);
+  if (m->ctr_pending) BXG_TRY(read_counters(m));
+  if (m->h_ctr->status) return map_status(m);
+  m->pending_base = m->h_ctr->scans_applied;
+  return BXG_OK;
+}
 
 // Confirm the asynchronous inserts enqueued on m since it was last confirmed
 // (blocking).  A pool overflow among them is repaired here: the device ran
@@ -1714,11 +1732,7 @@ int insert_async(bxg_map* m, const bxg_scan_desc& s) {
     m->extra.clear();
     return BXG_OK;
   }
-  if (m->pending.empty()) {
-    BXG_TRY(read_counters(m));
-    if (m->h_ctr->status) return map_status(m);
-    m->pending_base = m->h_ctr->scans_applied;
-  }
+  BXG_TRY(pending_begin(m));
   BXG_TRY(run_batch(c, &s, 1, 0));
   m->pending.push_back(s);
   if (m->pending.size() >= kMaxPendingInserts) BXG_TRY(settle(m));
@@ -1949,6 +1963,9 @@ int bxg_graph_set_scans(bxg_graph* g, const bxg_scan_desc* scans, uint32_t nscan
   bxg_ctx* c = g->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   if (nscans > g->max_scans) return fail(c, BXG_ERR_INVALID_ARG, "more scans than the graph holds");
+  // the previous frame's scans may still be pending on their maps, pointing
+  // at this graph's copies of their transforms: confirm them first
+  for (bxg_map* m : g->maps) BXG_TRY(settle(m));
   std::vector<ScanDesc> descs;
   uint64_t total = 0;
   BXG_TRY(build_descs(c, scans, nscans, 0u, descs, &total));
@@ -1970,9 +1987,15 @@ int bxg_graph_set_scans(bxg_graph* g, const bxg_scan_desc* scans, uint32_t nscan
   g->maps = maps;
   g->npoints.clear();
   g->ranges.clear();
+  g->scans.assign(scans, scans + nscans);
+  g->tfs.assign(nscans, std::array<float, 16>{});
   for (uint32_t s = 0; s < nscans; ++s) {
     g->npoints.push_back(scans[s].n);
     g->ranges.push_back(scans[s].max_range);
+    if (scans[s].sensor_to_world) {
+      std::memcpy(g->tfs[s].data(), scans[s].sensor_to_world, sizeof(float) * 16);
+      g->scans[s].sensor_to_world = g->tfs[s].data();
+    }
   }
   g->loaded = true;
   return BXG_OK;
@@ -1983,11 +2006,18 @@ int bxg_graph_launch(bxg_graph* g) {
   bxg_ctx* c = g->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   if (!g->loaded) return fail(c, BXG_ERR_INVALID_ARG, "no frame loaded");
-  for (bxg_map* m : g->maps) BXG_TRY(settle(m));
+  for (bxg_map* m : g->maps) BXG_TRY(pending_begin(m));
   for (size_t s = 0; s < g->maps.size(); ++s)
     BXG_TRY(ensure_room(g->maps[s], scan_leaf_estimate(g->maps[s], g->npoints[s], g->ranges[s]), 1024));
   BXG_HIP(c, hipGraphLaunch(g->exec, c->stream));
-  for (bxg_map* m : g->maps) BXG_TRY(snapshot_counters(m));
+  // each map keeps its scan of the frame until it is confirmed (settle),
+  // which re-runs it should the frame have overflowed the map's pools
+  for (size_t s = 0; s < g->maps.size(); ++s) {
+    bxg_map* m = g->maps[s];
+    BXG_TRY(snapshot_counters(m));
+    m->pending.push_back(g->scans[s]);
+    if (m->pending.size() >= kMaxPendingInserts) BXG_TRY(settle(m));
+  }
   return BXG_OK;
 }
 
@@ -1995,6 +2025,7 @@ int bxg_graph_destroy(bxg_graph* g) {
   if (!g) return BXG_ERR_INVALID_ARG;
   bxg_ctx* c = g->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
+  for (bxg_map* m : g->maps) settle(m);  // pending scans point at this graph's transforms
   hipStreamSynchronize(c->stream);
   if (g->exec) hipGraphExecDestroy(g->exec);
   if (g->graph) hipGraphDestroy(g->graph);

@@ -258,7 +258,12 @@ int bxg_classify_host(bxg_map* map, const int32_t* coords, uint64_t n, int8_t* s
 
 /* Capture the kernels of a batch of up to max_scans scans and max_points
  * points into a HIP graph; bxg_graph_set_scans loads a frame's scans into
- * the graph's descriptor buffer and bxg_graph_launch replays it. */
+ * the graph's descriptor buffer and bxg_graph_launch replays it.  A launch
+ * only enqueues; each map keeps its scan of the frame until it is confirmed
+ * (as for bxg_insert_point_cloud), which re-runs it should the frame have
+ * overflowed the map's pools.  bxg_graph_set_scans and bxg_graph_destroy
+ * confirm the previous frame's maps first; keep a frame's points valid
+ * until then. */
 int bxg_graph_create(bxg_ctx* ctx, uint32_t max_scans, uint64_t max_points, bxg_graph** out);
 int bxg_graph_set_scans(bxg_graph* g, const bxg_scan_desc* scans, uint32_t nscans);
 int bxg_graph_launch(bxg_graph* g);

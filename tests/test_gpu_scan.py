@@ -468,6 +468,42 @@ def test_batch_same_map_keeps_order(ctx):
         d.free()
 
 
+def test_graph_frames_replay_after_overflow(ctx):
+    """Two launches of a captured frame whose scans overflow their maps'
+    pools (two tiny scans first teach each map a small growth): confirming
+    the maps re-runs the skipped scans of both frames in order, and every map
+    equals the oracle's."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    scans = scenes.batch_scans(nmaps=4, lo=40_000, hi=120_000, seed=17)
+    maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+    models = [ora.ProbMap(0.1) for _ in scans]
+    for m, om, (p, o) in zip(maps, models, scans):
+        for _ in range(2):
+            m.insert_point_cloud(p[:300], o, 30.0)
+            om.insert_point_cloud(p[:300], o, 30.0)
+    caps = [m.stats()["leaf_capacity"] for m in maps]
+    dev = [ctx.to_device(p) for p, _ in scans]
+    g = bx.ScanGraph(ctx, 4, sum(len(p) for p, _ in scans))
+    g.set_scans([(m, d, o, 30.0) for m, d, (_, o) in zip(maps, dev, scans)])
+    for _ in range(2):
+        g.launch()
+        for om, (p, o) in zip(models, scans):
+            om.insert_point_cloud(p, o, 30.0)
+    for i, (m, om) in enumerate(zip(maps, models)):
+        assert m.sync() == 0
+        assert m.stats()["leaf_count"] > caps[i]  # the frame did not fit the pools it was given
+        assert_same_cells(_gpu_cells(m), om.cells(), f"map {i}")
+        assert m.update_count == om.update_count
+    g.close()
+    for m, om in zip(maps, models):
+        m.close()
+        om.close()
+    for d in dev:
+        d.free()
+
+
 def test_graph_replays_equal_eager(ctx):
     """A captured frame graph replayed four times (with eager work and
     synchronisation between replays) equals four eager batched inserts."""
