@@ -123,7 +123,11 @@ void srv_launch_read(bxg_server* s, SrvReq* r) {
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     hipStream_t st = s->streams[s->next_stream++ % s->streams.size()];
     r->t_admit = now_ns();
-    if (r->kind == kReqLookup) {
+    // inserts the map took outside the engine are confirmed first
+    rc = settle(m);
+    if (rc != BXG_OK) {
+      // reported below
+    } else if (r->kind == kReqLookup) {
       rc = launch_on(c, st, "k_lookup", k_lookup, lookup_blocks(r->n), kBlock, (const MapDev*)m->d, r->coords,
                      r->n, (uint8_t*)r->vals, r->found);
     } else {
@@ -164,8 +168,10 @@ size_t srv_launch_inserts(bxg_server* s, std::vector<SrvReq*>& ins) {
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     for (SrvReq* r : ins) {
       bxg_map* m = r->map;
-      // the map is idle, so its pools may move
-      const int g = ensure_room(m, scan_leaf_estimate(m, r->scan.n, r->scan.max_range), 1024);
+      // the map is idle, so its pools may move (inserts it took outside the
+      // engine are confirmed first)
+      int g = settle(m);
+      if (g == BXG_OK) g = ensure_room(m, scan_leaf_estimate(m, r->scan.n, r->scan.max_range), 1024);
       if (g == BXG_ERR_BUDGET) {
         back.push_back(r);
       } else if (g != BXG_OK) {
