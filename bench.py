@@ -330,7 +330,8 @@ def run_serving(ctx, bx, scenes, nreq=800, rate=1500.0, nmaps=16, seed=21):
            "deferred": stats["deferred"], "retried": stats["retried"]}
     for kind in ("insert", "lookup", "export"):
         lat = np.array([i["latency_ms"] for kd, i, _ in infos if kd == kind])
-        work = sum(n for kd, _, n in infos if kd == kind)
+        # an export's work is the number of cells it wrote, known only once it is done
+        work = sum(i["count"] if kd == "export" else n for kd, i, n in infos if kd == kind)
         if len(lat):
             out[kind] = {"count": int(len(lat)), "p50_ms": float(np.percentile(lat, 50)),
                          "p99_ms": float(np.percentile(lat, 99)),

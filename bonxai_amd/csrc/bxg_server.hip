@@ -255,7 +255,7 @@ bool srv_complete(bxg_server* s) {
           m->extra.clear();  // the addHit/MissPoint rays went with this scan
         }
       } else if (r->kind == kReqExport) {
-        r->count = (uint32_t)s->h_counts[r->slot];
+        r->count = s->h_counts[r->slot];
         if (st == BXG_OK && r->count > r->cap) st = BXG_ERR_CAPACITY;
       }
     }
