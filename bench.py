@@ -46,6 +46,7 @@ METRIC = "rays/s + voxel updates/s, batched vs plain per-scan on 1 MI355X; p50/p
 
 
 LOOKUP_BYTES = 28
+GATHER_TABLE = 64 << 20
 
 
 def log(*a):
@@ -177,6 +178,15 @@ def run_lookups(ctx, bx, scenes, reps=20):
     n, ms = ctx.profile_read("k_lookup")
     avg_ms = ms / max(n, 1)
     found = out_f.numpy()
+    # the memory system's rate for the hits' cell reads alone: as many scattered
+    # 4-byte reads from a 64 MiB table (the map's cells take 67 MB)
+    hits = int(found.sum())
+    ctx.profile_reset()
+    ctx.profile(True)
+    ctx.gather_probe(GATHER_TABLE, hits, reps=reps).free()
+    ctx.profile(False)
+    gn, gms = ctx.profile_read("k_gather_probe")
+    gather_ms = gms / max(gn, 1)
     for d in (dc, dv, dq, out_v, out_f):
         d.free()
     g.close()
@@ -184,6 +194,9 @@ def run_lookups(ctx, bx, scenes, reps=20):
     return {
         "lookups": nq, "lookup_ms": avg_ms, "lookups_per_s": nq / (avg_ms / 1e3),
         "hit_frac": float(found.mean()), "inserts": len(coords), "insert_ms": set_ms,
+        "gather_ceiling": {"reads": hits, "table_bytes": GATHER_TABLE, "ms": gather_ms,
+                           "reads_per_s": hits / (gather_ms / 1e3),
+                           "over_lookup_ms": gather_ms / avg_ms},
         "inserts_per_s": len(coords) / (set_ms / 1e3),
     }
 
@@ -460,7 +473,8 @@ def main():
     lk = None
     if not args.no_lookups:
         lk = run_lookups(ctx, bx, scenes)
-        log(f"lookups: {lk['lookup_ms']:.3f} ms per 8M")
+        log(f"lookups: {lk['lookup_ms']:.3f} ms per 8M; the hits' scattered reads alone "
+            f"{lk['gather_ceiling']['ms']:.3f} ms")
     c1 = c3 = None
     if not args.no_configs:
         c1 = run_box_scan(ctx, bx, scenes)

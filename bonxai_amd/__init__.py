@@ -166,6 +166,7 @@ ABI = [
     ("bxg_graph_destroy", _INT, [_P]),
     ("bxg_fp64_probe", _INT, [_P, _P, _P, _P, _U64, _P]),
     ("bxg_copy_probe", _INT, [_P, _P, _P, _U64]),
+    ("bxg_gather_probe", _INT, [_P, _P, _U64, _U64, _P]),
     ("bxg_server_create", _INT, [_P, ctypes.POINTER(ServerOptions), _PPTR]),
     ("bxg_server_destroy", _INT, [_P]),
     ("bxg_server_drain", _INT, [_P]),
@@ -360,6 +361,25 @@ class Context:
         finally:
             src.free()
             dst.free()
+
+    def gather_probe(self, table_bytes, n, reps=1, table=None):
+        """Run n scattered 4-byte reads over a table of table_bytes (a power of
+        two); returns the words read, as a device array the caller frees."""
+        own = table is None
+        if own:
+            table = self.array(table_bytes // 4, np.uint32)
+        out = self.array(n, np.uint32)
+        try:
+            for _ in range(reps):
+                _check(lib().bxg_gather_probe(self._h, table.ptr, table_bytes, n, out.ptr), self,
+                       "gather probe")
+        except BaseException:
+            out.free()
+            raise
+        finally:
+            if own:
+                table.free()
+        return out
 
     def fp64_probe(self, a, b, c):
         a = np.ascontiguousarray(a, np.float64)

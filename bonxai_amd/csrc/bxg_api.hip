@@ -2055,6 +2055,18 @@ int bxg_copy_probe(bxg_ctx* c, const void* src, void* dst, uint64_t bytes) {
   return ctx_sync(c);
 }
 
+int bxg_gather_probe(bxg_ctx* c, const void* table, uint64_t table_bytes, uint64_t n, void* out) {
+  const uint64_t words = table_bytes / 4;
+  if (!c || table_bytes % 4 || words == 0 || (words & (words - 1)) || (n && (!table || !out)))
+    return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (!n) return BXG_OK;
+  if (n > (uint64_t)UINT32_MAX * kBlock) return BXG_ERR_INVALID_ARG;
+  BXG_TRY(launch(c, "k_gather_probe", k_gather_probe, lookup_blocks(n), kBlock, (const uint32_t*)table,
+                 words - 1, n, (uint32_t*)out));
+  return ctx_sync(c);
+}
+
 int bxg_fp64_probe(bxg_ctx* c, const double* a, const double* b, const double* cc, uint64_t n,
                    double* out) {
   if (!c || (n && (!a || !b || !cc || !out))) return BXG_ERR_INVALID_ARG;

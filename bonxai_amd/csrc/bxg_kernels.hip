@@ -1629,6 +1629,34 @@ __global__ void __launch_bounds__(256) k_copy_probe(const uint4* __restrict__ sr
     dst[i] = src[i];
 }
 
+// n scattered 4-byte reads from a table of mask + 1 words (a power of two),
+// kLookups a thread as in k_lookup: the rate the memory system gives the cell
+// reads of a lookup batch whose coordinates come in no particular order
+__device__ __forceinline__ uint64_t probe_mix(uint64_t x) {
+  x ^= x >> 31;
+  x *= 0x7fb5d329728ea185ull;
+  x ^= x >> 27;
+  x *= 0x81dadef4bc2dd44dull;
+  return x ^ (x >> 33);
+}
+
+__global__ void __launch_bounds__(256) k_gather_probe(const uint32_t* __restrict__ table,
+                                                      uint64_t mask, uint64_t n,
+                                                      uint32_t* __restrict__ out) {
+  const uint64_t base = (uint64_t)blockIdx.x * blockDim.x * kLookups + threadIdx.x;
+  uint32_t v[kLookups];
+#pragma unroll
+  for (uint32_t j = 0; j < kLookups; ++j) {
+    const uint64_t i = base + (uint64_t)j * blockDim.x;
+    v[j] = i < n ? table[probe_mix(i) & mask] : 0u;
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < kLookups; ++j) {
+    const uint64_t i = base + (uint64_t)j * blockDim.x;
+    if (i < n) out[i] = v[j];
+  }
+}
+
 // ----------------------------------------------------- leaf page I/O
 
 // Copy whole leaves (mask words + cell bytes) named by their origin coords

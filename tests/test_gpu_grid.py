@@ -211,3 +211,31 @@ def test_coord_range_error(ctx):
     g.set_values(np.array([[5, 5, 5]], np.int32), np.ones(1, np.float32))
     assert g.active_cells_count() == 1
     g.close()
+
+
+def _probe_mix(i):
+    """The gather probe's index hash on uint64 (wrapping) arithmetic."""
+    x = i.astype(np.uint64)
+    x ^= x >> np.uint64(31)
+    x *= np.uint64(0x7fb5d329728ea185)
+    x ^= x >> np.uint64(27)
+    x *= np.uint64(0x81dadef4bc2dd44d)
+    return x ^ (x >> np.uint64(33))
+
+
+def test_gather_probe_reads_table(ctx):
+    """The scattered-read probe behind the lookup ceiling reads the words its
+    index hash names, including a ragged last workgroup; bad sizes are errors."""
+    import bonxai_amd as bx
+    words = 1 << 16
+    table_host = (np.arange(words, dtype=np.uint32) * np.uint32(2654435761))
+    table = ctx.to_device(table_host)
+    n = 100_003
+    out = ctx.gather_probe(words * 4, n, table=table)
+    idx = _probe_mix(np.arange(n, dtype=np.uint64)) & np.uint64(words - 1)
+    assert np.array_equal(out.numpy(), table_host[idx.astype(np.int64)])
+    out.free()
+    for bad in (words * 4 - 4, 0, 6):
+        with pytest.raises(bx.BonxaiError):
+            ctx.gather_probe(bad, 16, table=table)
+    table.free()
