@@ -86,8 +86,11 @@ struct bxg_ctx {
   // update pass and leaf-allocation pass over the rays, the workgroup size
   // of both, the most rays a wavefront takes per cursor atomic (rays per
   // lane = ray_chunk / 64), the endpoint-table capacity per point and the
-  // root-directory capacity per root (inverse load factors)
+  // root-directory capacity per root (inverse load factors); the update
+  // pass of a call with fewer points than its tuned grid has lanes runs on
+  // small_ray_blocks_per_cu workgroups per CU
   uint32_t ray_blocks_per_cu = 4;
+  uint32_t small_ray_blocks_per_cu = 4;
   uint32_t leaf_blocks_per_cu = 16;
   uint32_t ray_block = 256;
   uint32_t ray_chunk = 256;
@@ -664,11 +667,13 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
   const uint32_t most8 = (uint32_t)std::min<uint64_t>((grid_points * 8 + rb - 1) / rb, 1u << 30);
   // the tuned grid serves large calls (batched frames, lockstep batches); a
   // call with fewer points than that grid has lanes walks ray segments on a
-  // grid of 4 workgroups per CU, where more would only queue on one cursor
+  // grid of small_ray_blocks_per_cu workgroups per CU (4 by default: 16
+  // queue on one cursor; short rays into a fresh map gain from 8, long rays
+  // into a grown map from 2)
   const uint32_t gr_full = std::max<uint32_t>(1, cus * c->ray_blocks_per_cu);
   const uint32_t gr = grid_points >= (uint64_t)gr_full * rb
                           ? gr_full
-                          : std::max<uint32_t>(1, std::min<uint32_t>(cus * 4u, most8));
+                          : std::max<uint32_t>(1, std::min<uint32_t>(cus * c->small_ray_blocks_per_cu, most8));
   const uint32_t gl = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->leaf_blocks_per_cu, most));
   sc.chunk_max = c->ray_chunk;
   BXG_TRY(launch(c, "k_scan_classify", k_scan_classify, gp, kBlock, hdr, desc, sc));
@@ -1103,6 +1108,7 @@ struct TuningKey {
 const TuningKey kTuning[] = {
     {"ray_blocks_per_cu", &bxg_ctx::ray_blocks_per_cu, 1, 256, 1},
     {"leaf_blocks_per_cu", &bxg_ctx::leaf_blocks_per_cu, 1, 256, 1},
+    {"small_ray_blocks_per_cu", &bxg_ctx::small_ray_blocks_per_cu, 1, 64, 1},
     {"ray_block", &bxg_ctx::ray_block, 64, 256, 64},
     {"ray_chunk", &bxg_ctx::ray_chunk, 64, 1024, 64},
     {"ep_table_factor", &bxg_ctx::ep_table_factor, 2, 8, 1},

@@ -133,7 +133,11 @@ int bxg_profile_reset(bxg_ctx* ctx);
 int bxg_set_counting(bxg_ctx* ctx, int on);
 /* launch tuning of the context, set by the autotuner: "ray_blocks_per_cu"
  * and "leaf_blocks_per_cu" (1..256), the workgroups per CU of the
- * work-stealing update pass and leaf-allocation pass over the rays */
+ * work-stealing update pass and leaf-allocation pass over the rays;
+ * "small_ray_blocks_per_cu" (1..64, default 4), the update pass's workgroups
+ * per CU for a call with fewer points than the tuned grid has lanes;
+ * "ray_block", "ray_chunk", "ep_table_factor", "root_slack". Every setting
+ * gives the same maps. */
 int bxg_ctx_set_tuning(bxg_ctx* ctx, const char* key, int64_t value);
 int bxg_ctx_get_tuning(bxg_ctx* ctx, const char* key, int64_t* value);
 
