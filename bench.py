@@ -420,6 +420,24 @@ def cpu_baseline(scans, max_range, budget_s):
     return pts_done / dt, maps_done, pts_done, dt
 
 
+def cpu_reference_storage(scans, max_range):
+    """The same scans with the same integration over the reference's own
+    VoxelGrid<Cell> and Accessor (compiled from its headers into oracle/_ref),
+    one thread: how the oracle's speed relates to the reference's storage.
+    None where oracle/_ref was not built."""
+    import oracle
+    if not oracle.ref_available():
+        return None
+    t0 = time.perf_counter()
+    pts_done = 0
+    for pts, origin in scans:
+        m = oracle.RefProb(0.1)
+        m.insert_point_cloud(pts, origin, max_range)
+        m.close()
+        pts_done += len(pts)
+    return pts_done / (time.perf_counter() - t0)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -502,6 +520,7 @@ def main():
     from bonxai_amd import autotune
     ctx_tuning = {k: ctx.get_tuning(k) for k in autotune.CANDIDATES}
     cpu_rate, cpu_maps, cpu_pts, cpu_dt = cpu_baseline(scans, max_range, args.cpu_seconds)
+    ref_rate = cpu_reference_storage(scans[:cpu_maps], max_range)
     nthr = cpu_threads()
     mt_rate, mt_maps, mt_pts, mt_dt = cpu_baseline_threads(scans, max_range, args.cpu_seconds / 2, nthr)
     # visits as counted for CPU loops: every ray cell crossed, plus one
@@ -545,6 +564,11 @@ def main():
             "sample": f"the first {cpu_maps} scans of the frame ({cpu_pts} points, {cpu_dt:.1f} s), "
                       "oracle/bx_oracle.c -O2, one thread",
             "host": host_info(),
+            "reference_storage": None if ref_rate is None else {
+                "value": ref_rate, "unit": "rays/s", "cores": 1,
+                "sample": "the same scans, same integration, over the reference's own VoxelGrid<Cell> "
+                          "and Accessor (bonxai.hpp compiled into oracle/_ref)",
+                "port_over_reference": cpu_rate / ref_rate},
         },
         "cpu_baseline_threads": {
             "value": mt_rate, "unit": "rays/s", "cores": nthr, "kind": "port",

@@ -134,9 +134,7 @@ BXO_EXPORT bxo_grid* bxo_grid_create(double voxel_size, uint32_t inner_bits, uin
 
 static void bxo_free_leaf(bxo_leaf* l) {
   if (!l) return;
-  free(l->mask);
-  free(l->cells);
-  free(l);
+  free(l); /* header, mask and cells are one block */
 }
 
 static void bxo_free_inner(bxo_grid* g, bxo_inner* in) {
@@ -266,9 +264,12 @@ static bxo_leaf* bxo_get_leaf(bxo_accessor* a, bxo_coord c, int create) {
   const uint32_t ii = bxo_inner_index(g, c);
   if (create) {
     if (!bxo_bit_set(in->mask, ii)) {
-      bxo_leaf* l = (bxo_leaf*)calloc(1, sizeof(bxo_leaf));
-      l->mask = (uint64_t*)calloc(g->leaf_words, sizeof(uint64_t));
-      if (g->value_bytes) l->cells = (uint8_t*)calloc(g->leaf_size, g->value_bytes);
+      /* one block: the header, the mask words (inline in the leaf, as the
+       * reference's Mask is, mask.hpp:23,258) and the cells */
+      bxo_leaf* l = (bxo_leaf*)calloc(1, sizeof(bxo_leaf) + (size_t)g->leaf_words * sizeof(uint64_t) +
+                                             (size_t)g->leaf_size * g->value_bytes);
+      l->mask = (uint64_t*)(l + 1);
+      l->cells = g->value_bytes ? (uint8_t*)(l->mask + g->leaf_words) : NULL;
       in->child[ii] = l;
       g->leaf_count++;
     }
@@ -698,19 +699,45 @@ static void bxo_add_miss(bxo_prob* m, double x, double y, double z) {
 BXO_EXPORT void bxo_prob_add_hit(bxo_prob* m, const double p[3]) { bxo_add_hit(m, p[0], p[1], p[2]); }
 BXO_EXPORT void bxo_prob_add_miss(bxo_prob* m, const double p[3]) { bxo_add_miss(m, p[0], p[1], p[2]); }
 
-/* the clearPoint lambda of updateFreeCells, probabilistic_map.cpp:81-89 */
-static void bxo_clear_point(bxo_prob* m, bxo_accessor* a, bxo_coord c) {
-  uint8_t* cell = bxo_value(a, c, 1);
-  uint32_t w;
-  memcpy(&w, cell, 4);
-  m->st_visits++;
-  if (bxo_cell_id(w) != (int32_t)m->cur) {
-    int32_t p = (int32_t)((uint32_t)bxo_cell_p(w) + (uint32_t)m->miss);
-    if (p < m->cmin) p = m->cmin;
-    w = bxo_cell_make(p, m->cur);
-    memcpy(cell, &w, 4);
-    m->st_ray_updates++;
+/* The clearPoint lambda of updateFreeCells, probabilistic_map.cpp:81-89, over
+ * the cells of one ray.  The walk keeps the current leaf's mask and cell words
+ * and the scan's constants in locals: cell stores then cannot alias them, so
+ * the loop does not reload its state after every store (same results as going
+ * through bxo_value for every cell). */
+typedef struct {
+  bxo_coord key; /* inner key (leaf origin) of the cached leaf */
+  uint64_t* mask;
+  uint32_t* cells;
+  int valid;
+} bxo_ray_leaf;
+
+static inline uint32_t bxo_clear_cell(bxo_accessor* a, bxo_ray_leaf* rl, bxo_coord c,
+                                      int32_t cur, int32_t miss, int32_t cmin) {
+  const bxo_grid* g = a->g;
+  const bxo_coord ik = bxo_inner_key(g, c);
+  if (!rl->valid || !bxo_coord_eq(ik, rl->key)) {
+    bxo_leaf* l = bxo_cached_leaf(a, c, 1); /* Accessor::value(c, true), bonxai.hpp:469-494 */
+    rl->key = ik;
+    rl->mask = l->mask;
+    rl->cells = (uint32_t*)l->cells;
+    rl->valid = 1;
   }
+  const uint32_t li = bxo_leaf_index(g, c);
+  const uint64_t bit = 1ull << (li & 63);
+  uint32_t w = 0;
+  if (rl->mask[li >> 6] & bit) {
+    w = rl->cells[li];
+  } else {
+    rl->mask[li >> 6] |= bit; /* created value-initialised, bonxai.hpp:487-490 */
+    rl->cells[li] = 0u;
+  }
+  if (bxo_cell_id(w) != cur) {
+    int32_t p = (int32_t)((uint32_t)bxo_cell_p(w) + (uint32_t)miss);
+    if (p < cmin) p = cmin;
+    rl->cells[li] = bxo_cell_make(p, cur);
+    return 1;
+  }
+  return 0;
 }
 
 /* RayIterator, probabilistic_map.hpp:162-203: visits the origin cell and then
@@ -719,7 +746,9 @@ static void bxo_clear_point(bxo_prob* m, bxo_accessor* a, bxo_coord c) {
 static void bxo_ray(bxo_prob* m, bxo_accessor* a, bxo_coord o, bxo_coord e) {
   if (bxo_coord_eq(o, e)) return;
   m->st_rays++;
-  bxo_clear_point(m, a, o);
+  const int32_t cur = (int32_t)m->cur, miss = m->miss, cmin = m->cmin;
+  bxo_ray_leaf rl = {{0, 0, 0}, NULL, NULL, 0};
+  uint64_t upd = bxo_clear_cell(a, &rl, o, cur, miss, cmin);
   int32_t dx = e.x - o.x, dy = e.y - o.y, dz = e.z - o.z;
   const int32_t sx = dx < 0 ? -1 : 1, sy = dy < 0 ? -1 : 1, sz = dz < 0 ? -1 : 1;
   dx = dx < 0 ? -dx : dx;
@@ -745,8 +774,10 @@ static void bxo_ray(bxo_prob* m, bxo_accessor* a, bxo_coord o, bxo_coord e) {
       c.z += sz;
       ez -= mx;
     }
-    bxo_clear_point(m, a, c);
+    upd += bxo_clear_cell(a, &rl, c, cur, miss, cmin);
   }
+  m->st_visits += (uint64_t)mx;
+  m->st_ray_updates += upd;
 }
 
 /* updateFreeCells, probabilistic_map.cpp:77-106 */

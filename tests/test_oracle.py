@@ -352,3 +352,71 @@ def test_scan_integration_over_reference_storage():
         om.insert_point_cloud(pts, origin, 50.0)
     assert all(np.array_equal(a, b) for a, b in zip(rp.cells(), om.cells()))
     assert rp.update_count == om.update_count
+
+
+@needs_ref
+def test_cpu_restatement_speed_against_reference_storage(capsys):
+    """The CPU baseline is the restatement; this relates its speed to the
+    reference's own VoxelGrid storage (unordered_map roots, shared_ptr leaves,
+    block allocator) running the same integration on the same inputs: the
+    full 100k-point box scan, three 65k-ray LiDAR scans, and 200k clustered
+    setValue calls with 800k lookups. Results must agree. Each side runs three
+    times, alternating, and its fastest run counts; the ratios are printed
+    (run with -s)."""
+    import json
+    import time
+
+    def best(runs):
+        times = {"reference_storage_s": [], "restatement_s": []}
+        for _ in range(3):
+            for key, f in runs:
+                t = time.perf_counter()
+                after = f()
+                times[key].append(time.perf_counter() - t)
+                after()  # results are read outside the timed region
+        out = {k: min(v) for k, v in times.items()}
+        out["ratio"] = out["reference_storage_s"] / out["restatement_s"]
+        return out
+
+    box = scenes.box_scan()
+    lidar = list(scenes.lidar_sequence(3))
+
+    def scan_run(make, scans, rng):
+        def run():
+            m = make()
+            for pts, origin in scans:
+                m.insert_point_cloud(pts, origin, rng)
+
+            def after():
+                if len(cells) < 2:
+                    cells.append(m.cells())
+                m.close()
+            return after
+        return run
+
+    out = {}
+    for name, scans, res, rng in (("box_scan_100k", [box], 0.05, 10.0),
+                                  ("lidar_3_scans", lidar, 0.1, 50.0)):
+        cells = []
+        out[name] = best([("reference_storage_s", scan_run(lambda: oracle.RefProb(res), scans, rng)),
+                          ("restatement_s", scan_run(lambda: oracle.ProbMap(res), scans, rng))])
+        assert all(np.array_equal(a, b) for a, b in zip(cells[0], cells[1]))
+    coords = scenes.clusters(n=200_000)
+    vals = scenes.cluster_values(coords)
+    q = scenes.lookups(coords, n=800_000)
+    res = {}
+
+    def grid_run(key, make):
+        def run():
+            g = make()
+            g.set_values(coords, vals)
+            res[key] = g.lookup(q)
+            return g.close
+        return run
+
+    out["grid_200k_set_800k_lookup"] = best([
+        ("reference_storage_s", grid_run("r", lambda: oracle.RefGrid(0.05, np.float32))),
+        ("restatement_s", grid_run("g", lambda: oracle.Grid(0.05, dtype=np.float32)))])
+    assert np.array_equal(res["r"][1], res["g"][1]) and np.array_equal(res["r"][0], res["g"][0])
+    with capsys.disabled():
+        print("\ncpu timing, reference storage vs restatement:", json.dumps(out))
