@@ -366,6 +366,8 @@ class Context:
         """Run n scattered 4-byte reads over a table of table_bytes (a power of
         two); returns the words read, as a device array the caller frees."""
         own = table is None
+        if not own and table.nbytes < table_bytes:
+            raise ValueError("gather_probe: table_bytes exceeds the table's size")
         if own:
             table = self.array(table_bytes // 4, np.uint32)
         out = self.array(n, np.uint32)

@@ -238,4 +238,6 @@ def test_gather_probe_reads_table(ctx):
     for bad in (words * 4 - 4, 0, 6):
         with pytest.raises(bx.BonxaiError):
             ctx.gather_probe(bad, 16, table=table)
+    with pytest.raises(ValueError):
+        ctx.gather_probe(words * 8, 16, table=table)  # larger than the table: refused on the host
     table.free()
