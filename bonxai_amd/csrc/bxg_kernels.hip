@@ -305,21 +305,22 @@ struct RayWalk {
     kend = k1;
   }
   __device__ __forceinline__ bool done() const { return k >= kend; }
-  // move to cell k + 1
-  __device__ __forceinline__ void advance() {
+  // move to cell k + 1; h = ceil(mx / 2), so that 2 e >= mx <=> e >= h
+  // (one compare an axis instead of a shift and a compare)
+  __device__ __forceinline__ void advance(int32_t h) {
     ++k;
     ex += dx;
     ey += dy;
     ez += dz;
-    if (2 * ex >= mx) {
+    if (ex >= h) {
       cx += sx;
       ex -= mx;
     }
-    if (2 * ey >= mx) {
+    if (ey >= h) {
       cy += sy;
       ey -= mx;
     }
-    if (2 * ez >= mx) {
+    if (ez >= h) {
       cz += sz;
       ez -= mx;
     }
@@ -333,13 +334,17 @@ constexpr int kRun = 8;  // cells of one leaf handled per step of a ray
 __device__ __forceinline__ uint32_t collect_run(const MapView& m, RayWalk& w, uint32_t li[kRun]) {
   const int32_t lmask = ~((1 << m.lb) - 1);
   const int32_t lx = w.cx & lmask, ly = w.cy & lmask, lz = w.cz & lmask;
+  const int32_t h = (w.mx + 1) >> 1;
   uint32_t n = 0;
 #pragma unroll
   for (int j = 0; j < kRun; ++j) {
-    if (!w.done() && (w.cx & lmask) == lx && (w.cy & lmask) == ly && (w.cz & lmask) == lz) {
+    // one test without short-circuit branches: not done and still in the
+    // leaf (once false it stays false, since the walk no longer moves)
+    const bool in = (w.k < w.kend) & ((((w.cx ^ lx) | (w.cy ^ ly) | (w.cz ^ lz)) & lmask) == 0);
+    if (in) {
       li[j] = leaf_index(w.cx, w.cy, w.cz, m.lb);
       n = j + 1;
-      w.advance();
+      w.advance(h);
     }
   }
   return n;

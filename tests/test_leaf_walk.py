@@ -136,3 +136,17 @@ def test_segment_start_equals_stepping():
                 if 2 * er[i] >= m:
                     c[i] += s[i]
                     er[i] -= m
+
+
+def test_step_threshold_identity():
+    """RayWalk::advance tests e >= h with h = (m + 1) >> 1 in place of the
+    reference's 2 e >= m (probabilistic_map.hpp:162-203): equal for every
+    error the walk holds (-m <= e <= m after the add) and every length m."""
+    for m in range(1, 600):
+        h = (m + 1) >> 1
+        e = np.arange(-2 * m, 2 * m + 1)
+        assert np.array_equal(2 * e >= m, e >= h)
+    for m in (2**20 - 1, 2**20, 2**24 + 1, 2**26):
+        h = (m + 1) >> 1
+        e = np.array([-m, -1, 0, h - 2, h - 1, h, h + 1, m - 1, m], dtype=np.int64)
+        assert np.array_equal(2 * e >= m, e >= h)
