@@ -330,14 +330,18 @@ struct RayWalk {
 constexpr int kRun = 8;  // cells of one leaf handled per step of a ray
 
 // Advance a walk through the cells of its current leaf (at most kRun of
-// them), recording their leaf-local indices; returns how many.
+// them), recording their leaf-local indices; returns how many.  Slots past
+// the run hold the run's first index, so that every slot names a cell of the
+// leaf and its load needs no branch (it reads the first cell's line again).
 __device__ __forceinline__ uint32_t collect_run(const MapView& m, RayWalk& w, uint32_t li[kRun]) {
   const int32_t lmask = ~((1 << m.lb) - 1);
   const int32_t lx = w.cx & lmask, ly = w.cy & lmask, lz = w.cz & lmask;
   const int32_t h = (w.mx + 1) >> 1;
+  const uint32_t l0 = leaf_index(w.cx, w.cy, w.cz, m.lb);
   uint32_t n = 0;
 #pragma unroll
   for (int j = 0; j < kRun; ++j) {
+    li[j] = l0;
     // one test without short-circuit branches: not done and still in the
     // leaf (once false it stays false, since the walk no longer moves)
     const bool in = (w.k < w.kend) & ((((w.cx ^ lx) | (w.cy ^ ly) | (w.cz ^ lz)) & lmask) == 0);
@@ -496,7 +500,7 @@ __device__ __forceinline__ uint32_t apply_step(const MapView& m, RayWalk& w, Roo
   uint32_t v[kRun];
 #pragma unroll
   for (int j = 0; j < kRun; ++j)
-    if ((uint32_t)j < n) v[j] = base[li[j]];
+    v[j] = base[li[j]];  // slots j >= n re-read the run's first cell
   uint32_t nupd = 0;
 #pragma unroll
   for (int j = 0; j < kRun; ++j) {
