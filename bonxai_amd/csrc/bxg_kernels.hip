@@ -74,6 +74,20 @@ __device__ __forceinline__ uint32_t max32(uint32_t* p, uint32_t v) {
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanes_below() { return (1ull << lane_id()) - 1ull; }
 
+// Minimum over the 64 lanes of a wavefront, every lane active: DPP moves
+// within quads (quad_perm [1,0,3,2], [2,3,0,1]) and rows (row_ror 4, 8), then
+// row_bcast 15 into rows 1 and 3 and row_bcast 31 into rows 2 and 3, so that
+// lane 63 holds the minimum; no LDS round trips.
+__device__ __forceinline__ int32_t wave_min(int32_t v) {
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xF, 0xF, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xF, 0xF, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xA, 0xF, false));
+  v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xC, 0xF, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
 // Append from the active lanes of a wavefront with one atomic per wavefront;
 // returns this lane's slot (valid only where pred is true).
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
@@ -943,14 +957,23 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
     // otherwise; measured: the update pass about 8 % faster)
     const int32_t kk = kMode == 0 ? lw.k : w.k;
     int32_t kmin = have ? kk : 0x7fffffff;
-    if (lockstep) {
+    bool go = have;
+    if (lockstep && kMode == 0) {
+      // the leaf pass: DPP minimum (it also frees the registers that keep
+      // the pass at 5 waves a SIMD; the update pass measured no gain); a
+      // wavefront with no lane free to move lets every lane with a ray move
+      kmin = wave_min(kmin);
+      go = have && kk < kmin + kSyncWindow;
+      if (__ballot(go) == 0) go = have;
+    } else if (lockstep) {
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
         const int32_t t = __shfl_xor(kmin, o);
         kmin = t < kmin ? t : kmin;
       }
+      go = have && kk < kmin + kSyncWindow;
     }
-    if (have && (!lockstep || kk < kmin + kSyncWindow)) {
+    if (go) {
       if (kMode == 0) {
         leaves_step(m, lw, rc);
       } else {
