@@ -225,7 +225,12 @@ __device__ __forceinline__ uint32_t find_leaf(const MapView& m, int32_t x, int32
 struct RootCache {
   int32_t rx, ry, rz, slot;
   bool valid;
-  __device__ __forceinline__ void reset() { valid = false; }
+  // fields set, so that the cache test below may read them without branches
+  __device__ __forceinline__ void reset() {
+    rx = ry = rz = 0;
+    slot = -1;
+    valid = false;
+  }
 };
 
 // slot of the root holding (x, y, z): looked up (kInsert false) or created
@@ -235,7 +240,7 @@ __device__ __forceinline__ int32_t root_slot(const MapView& m, RootCache& rc, in
                                              int32_t z) {
   const int32_t rmask = ~((1 << m.log2n) - 1);
   const int32_t rx = x & rmask, ry = y & rmask, rz = z & rmask;
-  if (!rc.valid || rx != rc.rx || ry != rc.ry || rz != rc.rz) {
+  if (!rc.valid | (((rx ^ rc.rx) | (ry ^ rc.ry) | (rz ^ rc.rz)) != 0)) {
     uint64_t key;
     int32_t s = -1;
     if (pack_root(x, y, z, m.log2n, &key)) s = kInsert ? root_insert(m, key) : root_find(m, key);
@@ -518,7 +523,8 @@ __device__ __forceinline__ uint32_t apply_step(const MapView& m, RayWalk& w, Roo
   uint32_t nupd = 0;
 #pragma unroll
   for (int j = 0; j < kRun; ++j) {
-    if ((uint32_t)j < n && cell_id(v[j]) != cur) {
+    const bool upd = ((uint32_t)j < n) & (cell_id(v[j]) != cur);  // one branch, not two
+    if (upd) {
       uint32_t old = v[j];
       if (kCount) {
         for (;;) {
