@@ -851,7 +851,11 @@ __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const
 }
 
 constexpr uint32_t kChunk = 256;  // default of ScanScratch::chunk_max
-constexpr int32_t kSyncWindow = 16;  // cells a lane may run ahead of its wavefront's slowest ray
+// cells a lane may run ahead of its wavefront's slowest ray in lockstep
+// batches: update pass and leaf pass (measured on 64 scans a call: 8 beats
+// 4 and 16 for the update pass, 32 beats 16 for the leaf pass)
+constexpr int32_t kSyncWindowCells = 8;
+constexpr int32_t kSyncWindowLeaves = 32;
 constexpr uint32_t kLockstepRays = 1;  // rays a lane from which batches run in lockstep
 constexpr uint32_t kMaxSegLog = 3;     // at most 8 segments a ray when rays are fewer than lanes
 
@@ -957,7 +961,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       chunk_pos += take;
     }
     if (__ballot(have) == 0) break;
-    // lockstep: a lane more than kSyncWindow cells ahead of the wavefront's
+    // lockstep: a lane more than the window's cells ahead of the wavefront's
     // slowest ray waits a step, so the batch stays at about one distance
     // from the sensor (runs end at leaf borders, so rays drift apart
     // otherwise; measured: the update pass about 8 % faster)
@@ -969,7 +973,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       // the pass at 5 waves a SIMD; the update pass measured no gain); a
       // wavefront with no lane free to move lets every lane with a ray move
       kmin = wave_min(kmin);
-      go = have && kk < kmin + kSyncWindow;
+      go = have && kk < kmin + kSyncWindowLeaves;
       if (__ballot(go) == 0) go = have;
     } else if (lockstep) {
 #pragma unroll
@@ -977,7 +981,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
         const int32_t t = __shfl_xor(kmin, o);
         kmin = t < kmin ? t : kmin;
       }
-      go = have && kk < kmin + kSyncWindow;
+      go = have && kk < kmin + kSyncWindowCells;
     }
     if (go) {
       if (kMode == 0) {
