@@ -6,17 +6,19 @@ CPU restatement timed beside it.
 
 Workload (one "step" = one frame): 64 independent maps at 0.1 m voxels, each
 receiving one spinning-LiDAR scan of a seeded box scene, point counts
-log-uniform in 10k..500k (about 8M points a frame), max range 30 m.  Inputs
-are resident in HBM before the timed region; the same frame is integrated
-every step into maps that persist (steady state: every touched cell takes an
-update each frame, no cell is frozen).
+log-uniform in 10k..500k (about 6.9M points a frame), max range 30 m.  Every
+timed frame goes into 64 EMPTY maps (the workload as stated: every cell it
+touches is created, as in the reference's first insert into a new map); the
+map sets are emptied outside the timed events.  Inputs are resident in HBM
+before the timed region.  The same frame re-inserted into maps that already
+hold it is reported beside it as "steady".
 
 Prints one JSON line.  Extra fields: the plain per-scan rate, the
 batched/plain ratio, voxel updates/s, per-frame p50/p99 latency, the
-roofline of the ray kernel (k_scan_rays) and of the bulk lookup kernel
-(VoxelGrid<float>, 2M inserts / 8M lookups), the serving form (16 resident
-maps, a Poisson stream of depth-frame inserts, lookups and exports; p50/p99
-per request type), and the CPU baseline.
+roofline of the ray kernel (k_scan_rays) and of the bulk lookup and insert
+kernels (VoxelGrid<float>, 2M inserts / 8M lookups), the serving form (16
+resident maps, a Poisson stream of depth-frame inserts, lookups and exports;
+p50/p99 per request type), and the CPU baselines.
 """
 import argparse
 import json
@@ -55,96 +57,206 @@ def log(*a):
 
 def frame_stats(maps):
     st = [m.stats() for m in maps]
-    return (sum(x["last_rays"] for x in st), sum(x["last_ray_updates"] for x in st),
-            sum(x["last_ray_visits"] for x in st))
+    return {"rays": sum(x["last_rays"] for x in st), "ray_updates": sum(x["last_ray_updates"] for x in st),
+            "ray_visits": sum(x["last_ray_visits"] for x in st), "leaves": sum(x["leaf_count"] for x in st)}
 
 
-def run_batched(ctx, bx, scans, dev, steps, warmup, max_range):
-    maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
-    descs = [(m, d, o, max_range) for m, d, (_, o) in zip(maps, dev, scans)]
-    # first frame through the blocking batched call: it sizes every pool
-    bx.insert_batch(ctx, descs)
+def reset_maps(ctx, bx, maps):
+    """Empty the maps as new ones are (no roots, no leaves, update_count 1);
+    their pools keep their capacity, so a frame into them allocates no device
+    memory."""
     for m in maps:
+        m.clear(bx.CLEAR_MEMORY)
+        m.update_count = 1
+    ctx.sync()
+
+
+def make_map_sets(ctx, bx, scans, dev, max_range, nsets, budget_bytes):
+    """Up to nsets sets of one map per scan (as many as fit in budget_bytes),
+    each pool reserved from a sizing pass of the frame (1.25x its leaves), so
+    that no timed frame allocates memory or overflows a pool."""
+    sizing = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+    bx.insert_batch(ctx, [(m, d, o, max_range) for m, d, (_, o) in zip(sizing, dev, scans)])
+    need = []
+    for m in sizing:
         st = m.stats()
-        m.reserve(int(st["leaf_count"] * 1.25) + 8192, int(st["root_count"] * 2) + 1024)
-    # one counted frame: exact rays and cell updates per frame
-    ctx.counting(True)
-    bx.insert_batch(ctx, descs)
-    ctx.counting(False)
-    rays, ray_upd, visits = frame_stats(maps)
+        need.append((st["leaf_count"], st["root_count"]))
+        m.close()
+    # leaf: 8 mask words, 512 cells of 4 bytes, origin; root: key + 64 entries
+    per_set = sum((int(nl * 1.25) + 8192) * (64 + 2048 + 16) + 2 * (int(nr * 2) + 1024) * 264
+                  for nl, nr in need)
+    nsets = max(1, min(nsets, int(budget_bytes // per_set)))
+    sets = []
+    for _ in range(nsets):
+        maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+        for m, (nl, nr) in zip(maps, need):
+            m.reserve(int(nl * 1.25) + 8192, int(nr * 2) + 1024)
+        sets.append(maps)
+    ctx.sync()
+    return sets, sum(nl for nl, _ in need)
+
+
+def replays(sets):
+    return sum(m.stats()["replays"] for maps in sets for m in maps)
+
+
+def check_maps(maps, what):
+    for m in maps:
+        if m.sync() != 0:
+            raise RuntimeError(f"a map failed during the {what}")
+
+
+def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets):
+    """The batched form, one captured graph launch per frame of 64 scans.
+    Fresh frames (the workload as stated, and `value`): each timed step
+    integrates the frame into a set of 64 empty maps; up to len(sets) steps run
+    back to back between two events, the sets being emptied beforehand outside
+    the events.  Steady frames: the same frame re-inserted back to back into
+    maps that already hold it."""
+    descs = [[(m, d, o, max_range) for m, d, (_, o) in zip(maps, dev, scans)] for maps in sets]
+    frames = [bx.ScanGraph.prepare(d) for d in descs]
     total_points = sum(len(p) for p, _ in scans)
+    nsets = len(sets)
+    replays0 = replays(sets)
+    # counted frames: exact rays, cell updates and visits, fresh and steady
+    reset_maps(ctx, bx, sets[0])
+    ctx.counting(True)
+    bx.insert_batch(ctx, descs[0])
+    fresh_counts = frame_stats(sets[0])
+    bx.insert_batch(ctx, descs[0])
+    steady_counts = frame_stats(sets[0])
+    ctx.counting(False)
     g = bx.ScanGraph(ctx, len(scans), total_points)
-    g.set_scans(descs)
+    # fresh frames, warm-up then timed
+    for ms in sets:
+        reset_maps(ctx, bx, ms)
+    for k in range(min(warmup, nsets)):
+        g.set_scans(frames[k])
+        g.launch()
+    ctx.sync()
+    fresh_ms = 0.0
+    done = rounds = 0
+    while done < steps:
+        n = min(nsets, steps - done)
+        for ms in sets[:n]:
+            reset_maps(ctx, bx, ms)
+        ctx.event(0)
+        for k in range(n):
+            g.set_scans(frames[k])
+            g.launch()
+        ctx.event(1)
+        fresh_ms += ctx.elapsed_ms(0, 1)
+        done += n
+        rounds += 1
+    for ms in sets:
+        check_maps(ms, "timed fresh frames")
+    fresh_replays = replays(sets) - replays0
+    # per-frame latency of fresh frames, one frame between two events
+    lat_fresh = []
+    for k in range(max(steps, 10)):
+        ms = sets[k % nsets]
+        reset_maps(ctx, bx, ms)
+        ctx.event(2)
+        g.set_scans(frames[k % nsets])
+        g.launch()
+        ctx.event(3)
+        lat_fresh.append(ctx.elapsed_ms(2, 3))
+    # steady frames into set 0, which holds the frame after the loop above
+    check_maps(sets[0], "latency frames")
+    g.set_scans(frames[0])
     for _ in range(warmup):
         g.launch()
     ctx.sync()
-    lat = []
     ctx.event(0)
     for _ in range(steps):
         g.launch()
     ctx.event(1)
-    total_ms = ctx.elapsed_ms(0, 1)
-    for m in maps:
-        if m.sync() != 0:
-            raise RuntimeError("a map failed during the timed frames")
-    # per-frame latency: frames timed one at a time
-    for k in range(max(steps, 10)):
+    steady_ms = ctx.elapsed_ms(0, 1)
+    check_maps(sets[0], "steady frames")
+    lat_steady = []
+    for _ in range(max(steps, 10)):
         ctx.event(2)
         g.launch()
         ctx.event(3)
-        lat.append(ctx.elapsed_ms(2, 3))
-    # kernel durations of the same frame, eager launches with per-kernel events
+        lat_steady.append(ctx.elapsed_ms(2, 3))
+    check_maps(sets[0], "steady latency frames")
+    all_replays = replays(sets) - replays0
+    # kernel durations, eager launches with per-kernel events
+    names = ("k_scan_begin", "k_scan_classify", "k_scan_winners", "k_scan_bucket_reduce",
+             "k_scan_bucket_scan", "k_scan_scatter", "k_scan_ray_leaves", "k_scan_endpoints",
+             "k_scan_rays", "k_scan_new_masks", "k_scan_finalize")
+    nprof = 3
+    ctx.profile_reset()
+    for _ in range(nprof):
+        reset_maps(ctx, bx, sets[0])
+        ctx.profile(True)
+        bx.insert_batch(ctx, descs[0])
+        ctx.profile(False)
+    kern_fresh = {k: ctx.profile_read(k)[1] / max(ctx.profile_read(k)[0], 1) for k in names}
     ctx.profile_reset()
     ctx.profile(True)
-    nprof = 3
     for _ in range(nprof):
-        bx.insert_batch(ctx, descs)
+        bx.insert_batch(ctx, descs[0])
     ctx.profile(False)
-    kern = {}
-    for name in ("k_scan_classify", "k_scan_winners", "k_scan_bucket_reduce", "k_scan_bucket_scan",
-                 "k_scan_scatter", "k_scan_ray_leaves", "k_scan_endpoints", "k_scan_rays",
-                 "k_scan_finalize"):
-        n, ms = ctx.profile_read(name)
-        kern[name] = ms / max(n, 1)
+    kern_steady = {k: ctx.profile_read(k)[1] / max(ctx.profile_read(k)[0], 1) for k in names}
     g.close()
-    for m in maps:
-        m.close()
     return {
-        "total_ms": total_ms, "lat": lat, "rays_traced": rays, "ray_updates": ray_upd,
-        "ray_visits": visits,
-        "kernels_ms": kern, "points": total_points,
+        "points": total_points, "fresh_ms": fresh_ms, "fresh_rounds": rounds, "steady_ms": steady_ms,
+        "lat_fresh": lat_fresh, "lat_steady": lat_steady, "fresh_counts": fresh_counts,
+        "steady_counts": steady_counts, "kern_fresh": kern_fresh, "kern_steady": kern_steady,
+        "replays_timed": fresh_replays, "replays_all": all_replays,
     }
 
 
-def run_plain(ctx, bx, scans, dev, steps, warmup, max_range):
-    maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
-    bx.insert_batch(ctx, [(m, d, o, max_range) for m, d, (_, o) in zip(maps, dev, scans)])
-    for m in maps:
-        st = m.stats()
-        m.reserve(int(st["leaf_count"] * 1.25) + 8192, int(st["root_count"] * 2) + 1024)
-    for _ in range(warmup):
-        for m, d, (_, o) in zip(maps, dev, scans):
+def run_plain(ctx, bx, scans, dev, steps, warmup, max_range, sets):
+    """The plain per-scan form, one insert call per scan (64 a frame), into
+    fresh maps (as run_batched) and re-inserted into maps holding the frame."""
+    nsets = len(sets)
+    for ms in sets:
+        reset_maps(ctx, bx, ms)
+    for k in range(min(warmup, nsets)):
+        for m, d, (_, o) in zip(sets[k], dev, scans):
             m.insert_point_cloud(d, o, max_range, sync=False)
     ctx.sync()
-    ctx.event(0)
-    for _ in range(steps):
-        for m, d, (_, o) in zip(maps, dev, scans):
-            m.insert_point_cloud(d, o, max_range, sync=False)
-    ctx.event(1)
-    total_ms = ctx.elapsed_ms(0, 1)
-    for m in maps:
-        if m.sync() != 0:
-            raise RuntimeError("a map failed during the plain frames")
-    # per-scan latency of the plain form (each scan synchronised)
+    fresh_ms = 0.0
+    done = 0
+    while done < steps:
+        n = min(nsets, steps - done)
+        for ms in sets[:n]:
+            reset_maps(ctx, bx, ms)
+        ctx.event(0)
+        for k in range(n):
+            for m, d, (_, o) in zip(sets[k], dev, scans):
+                m.insert_point_cloud(d, o, max_range, sync=False)
+        ctx.event(1)
+        fresh_ms += ctx.elapsed_ms(0, 1)
+        done += n
+    for ms in sets:
+        check_maps(ms, "plain fresh frames")
+    # per-scan latency, fresh maps (each insert between two events)
     lat = []
-    for m, d, (_, o) in zip(maps[:16], dev[:16], scans[:16]):
+    reset_maps(ctx, bx, sets[0])
+    for m, d, (_, o) in zip(sets[0][:16], dev[:16], scans[:16]):
         ctx.event(2)
         m.insert_point_cloud(d, o, max_range, sync=False)
         ctx.event(3)
         lat.append(ctx.elapsed_ms(2, 3))
-    for m in maps:
-        m.close()
-    return {"total_ms": total_ms, "lat": lat}
+    # steady: set 0 holds the frame (or part of it): fill it, then re-insert
+    reset_maps(ctx, bx, sets[0])
+    for m, d, (_, o) in zip(sets[0], dev, scans):
+        m.insert_point_cloud(d, o, max_range, sync=False)
+    for _ in range(warmup):
+        for m, d, (_, o) in zip(sets[0], dev, scans):
+            m.insert_point_cloud(d, o, max_range, sync=False)
+    ctx.sync()
+    ctx.event(0)
+    for _ in range(steps):
+        for m, d, (_, o) in zip(sets[0], dev, scans):
+            m.insert_point_cloud(d, o, max_range, sync=False)
+    ctx.event(1)
+    steady_ms = ctx.elapsed_ms(0, 1)
+    check_maps(sets[0], "plain steady frames")
+    return {"fresh_ms": fresh_ms, "steady_ms": steady_ms, "lat": lat}
 
 
 def run_lookups(ctx, bx, scenes, reps=20):
@@ -444,6 +556,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--maps", type=int, default=64)
+    ap.add_argument("--map-sets", type=int, default=20,
+                    help="sets of empty maps timed back to back (at most --steps)")
+    ap.add_argument("--map-set-gib", type=int, default=96,
+                    help="device memory the map sets may take")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-lookups", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -482,12 +598,21 @@ def main():
         ctx.set_tuning("ray_blocks_per_cu", args.ray_blocks_per_cu)
     dev = [ctx.to_device(p) for p, _ in scans]
 
-    b = run_batched(ctx, bx, scans, dev, args.steps, args.warmup, max_range)
-    log(f"batched: {b['total_ms'] / args.steps:.3f} ms/frame")
+    t0 = time.time()
+    sets, frame_leaves = make_map_sets(ctx, bx, scans, dev, max_range, min(args.steps, args.map_sets),
+                                       args.map_set_gib << 30)
+    log(f"{len(sets)} sets of {len(scans)} maps ({frame_leaves} leaves a fresh frame) in {time.time() - t0:.1f}s")
+    b = run_batched(ctx, bx, scans, dev, args.steps, args.warmup, max_range, sets)
+    log(f"batched: fresh {b['fresh_ms'] / args.steps:.3f} ms/frame, steady {b['steady_ms'] / args.steps:.3f} "
+        f"ms/frame, replays {b['replays_all']}")
     p = None
     if not args.no_plain:
-        p = run_plain(ctx, bx, scans, dev, args.steps, args.warmup, max_range)
-        log(f"plain: {p['total_ms'] / args.steps:.3f} ms/frame")
+        p = run_plain(ctx, bx, scans, dev, args.steps, args.warmup, max_range, sets)
+        log(f"plain: fresh {p['fresh_ms'] / args.steps:.3f} ms/frame, steady {p['steady_ms'] / args.steps:.3f}")
+    for ms in sets:
+        for m in ms:
+            m.close()
+    del sets
     lk = None
     if not args.no_lookups:
         lk = run_lookups(ctx, bx, scenes)
@@ -506,17 +631,25 @@ def main():
             f"{sv['insert']['p50_ms']:.2f}/{sv['insert']['p99_ms']:.2f} ms")
 
     npts = b["points"]
-    ms_step = b["total_ms"] / args.steps
+    fc, sc_ = b["fresh_counts"], b["steady_counts"]
+    ms_step = b["fresh_ms"] / args.steps
     rays_s = npts / (ms_step / 1e3)
-    # voxel updates per frame: endpoint updates (one per ray) + ray updates
-    upd = b["rays_traced"] + b["ray_updates"]
-    upd_s = upd / (ms_step / 1e3)
+    ms_steady = b["steady_ms"] / args.steps
+    # voxel updates per frame: endpoint updates (one per traced ray) + ray
+    # updates (every distinct cell a ray crossed whose update_id was not cur)
+    upd = fc["rays"] + fc["ray_updates"]
+    upd_steady = sc_["rays"] + sc_["ray_updates"]
     # algorithmic bytes of one k_scan_rays launch: per ray its 16-byte record
     # and 4-byte list entry; per updated cell one 4-byte read and one 4-byte
     # write (every distinct cell once)
-    k_ms = b["kernels_ms"]["k_scan_rays"]
-    alg_bytes = 20 * b["rays_traced"] + 8 * b["ray_updates"]
+    k_ms = b["kern_fresh"]["k_scan_rays"]
+    alg_bytes = 20 * fc["rays"] + 8 * fc["ray_updates"]
     achieved = alg_bytes / (k_ms / 1e3) / 1e9
+    k_ms_st = b["kern_steady"]["k_scan_rays"]
+    alg_bytes_st = 20 * sc_["rays"] + 8 * sc_["ray_updates"]
+    # the whole frame on the integration model: 12 B a point read, 8 B per
+    # updated cell, 64 B of mask words per new leaf
+    frame_bytes = 12 * npts + 8 * upd + 64 * fc["leaves"]
     from bonxai_amd import autotune
     ctx_tuning = {k: ctx.get_tuning(k) for k in autotune.CANDIDATES}
     cpu_rate, cpu_maps, cpu_pts, cpu_dt = cpu_baseline(scans, max_range, args.cpu_seconds)
@@ -525,8 +658,9 @@ def main():
     mt_rate, mt_maps, mt_pts, mt_dt = cpu_baseline_threads(scans, max_range, args.cpu_seconds / 2, nthr)
     # visits as counted for CPU loops: every ray cell crossed, plus one
     # endpoint visit per point
-    visits = b["ray_visits"] + npts
-    lat = np.array(b["lat"])
+    visits = fc["ray_visits"] + npts
+    lat = np.array(b["lat_fresh"])
+    lat_st = np.array(b["lat_steady"])
     out = {
         "metric": METRIC,
         "value": rays_s,
@@ -541,28 +675,55 @@ def main():
         "dtype": "int32",
         "data": "synthetic",
         "config": {
-            "workload": f"batched insertPointCloud: {len(scans)} maps x 1 LiDAR scan each "
+            "workload": f"batched insertPointCloud: {len(scans)} empty maps x 1 LiDAR scan each "
                         "(log-uniform 10k-500k points, box scene), voxel 0.1 m, max range 30 m, "
                         "one captured graph per frame",
             "maps": len(scans), "points_per_frame": npts, "voxel": 0.1, "max_range": max_range,
-            "rays_traced_per_frame": b["rays_traced"], "ray_updates_per_frame": b["ray_updates"],
-            "ray_visits_per_frame": b["ray_visits"], "input_sha256": scenes.digest_scans(scans),
+            "rays_traced_per_frame": fc["rays"], "ray_updates_per_frame": fc["ray_updates"],
+            "ray_visits_per_frame": fc["ray_visits"], "leaves_per_frame": fc["leaves"],
+            "input_sha256": scenes.digest_scans(scans),
+            "timing": f"{args.steps} frames in {b['fresh_rounds']} timed region(s) of back-to-back graph "
+                      "launches, each frame into its own set of empty maps; the sets are emptied between "
+                      "regions, outside the timed events",
             "tuning": {k: ctx_tuning[k] for k in sorted(ctx_tuning)},
         },
-        "voxel_updates_per_s": upd_s,
+        "rays_per_s_note": "value counts input points (one ray each as insertPointCloud takes them); "
+                           "traced_rays_per_s counts the distinct endpoint voxels whose ray was walked",
+        "traced_rays_per_s": fc["rays"] / (ms_step / 1e3),
+        "voxel_updates_per_s": upd / (ms_step / 1e3),
         "visits_per_s": visits / (ms_step / 1e3),
         "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-        "kernels_ms": b["kernels_ms"],
+        "replays_after_timed_frames": b["replays_timed"],
+        "kernels_ms": b["kern_fresh"],
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_scan_rays"),
             "kernel": "k_scan_rays",
             "alg_bytes_per_launch": alg_bytes,
         },
+        "frame_roofline": {
+            "bound": "hbm", "alg_bytes_per_frame": frame_bytes,
+            "achieved": frame_bytes / (ms_step / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": frame_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
+        },
+        "steady": {
+            "what": "the same frame re-inserted back to back into the maps that hold it",
+            "rays_per_s": npts / (ms_steady / 1e3), "ms_per_frame": ms_steady,
+            "voxel_updates_per_s": upd_steady / (ms_steady / 1e3),
+            "ray_updates_per_frame": sc_["ray_updates"],
+            "frame_latency_ms": {"p50": float(np.percentile(lat_st, 50)),
+                                 "p99": float(np.percentile(lat_st, 99))},
+            "kernels_ms": b["kern_steady"],
+            "roofline": {"bound": "hbm", "kernel": "k_scan_rays", "alg_bytes_per_launch": alg_bytes_st,
+                         "achieved": alg_bytes_st / (k_ms_st / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "frac": alg_bytes_st / (k_ms_st / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "traffic": measured_traffic("k_scan_rays_steady")},
+            "replays": b["replays_all"] - b["replays_timed"],
+        },
         "cpu_baseline": {
             "value": cpu_rate, "unit": "rays/s", "cores": 1, "kind": "port",
-            "sample": f"the first {cpu_maps} scans of the frame ({cpu_pts} points, {cpu_dt:.1f} s), "
-                      "oracle/bx_oracle.c -O2, one thread",
+            "sample": f"the first {cpu_maps} scans of the frame ({cpu_pts} points, {cpu_dt:.1f} s), each "
+                      "into an empty map, oracle/bx_oracle.c -O2, one thread",
             "host": host_info(),
             "reference_storage": None if ref_rate is None else {
                 "value": ref_rate, "unit": "rays/s", "cores": 1,
@@ -573,15 +734,18 @@ def main():
         "cpu_baseline_threads": {
             "value": mt_rate, "unit": "rays/s", "cores": nthr, "kind": "port",
             "sample": f"the first {mt_maps} scans of the frame ({mt_pts} points, {mt_dt:.1f} s), "
-                      f"one map per thread, {nthr} threads",
+                      f"one empty map per thread, {nthr} threads",
         },
     }
     if p is not None:
-        plain_rays_s = npts / (p["total_ms"] / args.steps / 1e3)
-        out["plain"] = {"rays_per_s": plain_rays_s, "ms_per_frame": p["total_ms"] / args.steps,
+        plain_rays_s = npts / (p["fresh_ms"] / args.steps / 1e3)
+        plain_steady = npts / (p["steady_ms"] / args.steps / 1e3)
+        out["plain"] = {"rays_per_s": plain_rays_s, "ms_per_frame": p["fresh_ms"] / args.steps,
+                        "steady_rays_per_s": plain_steady, "steady_ms_per_frame": p["steady_ms"] / args.steps,
                         "scan_latency_ms": {"p50": float(np.percentile(p["lat"], 50)),
                                             "p99": float(np.percentile(p["lat"], 99))}}
         out["batched_over_plain"] = rays_s / plain_rays_s
+        out["steady"]["batched_over_plain"] = (npts / (ms_steady / 1e3)) / plain_steady
     if lk is not None:
         # algorithmic bytes of a lookup, the declared model: 12-byte coordinate,
         # 8-byte mask word and 4-byte cell read, 4-byte value written (root and

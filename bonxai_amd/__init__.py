@@ -74,7 +74,7 @@ class MapStats(ctypes.Structure):
                 ("scans_applied", ctypes.c_uint64), ("device_bytes", ctypes.c_uint64),
                 ("last_rays", ctypes.c_uint64), ("last_ray_updates", ctypes.c_uint64),
                 ("update_count", ctypes.c_uint32), ("status", ctypes.c_uint32),
-                ("last_ray_visits", ctypes.c_uint64)]
+                ("last_ray_visits", ctypes.c_uint64), ("replays", ctypes.c_uint64)]
 
 
 class ScanDesc(ctypes.Structure):
@@ -713,9 +713,22 @@ class ScanGraph:
         _check(lib().bxg_graph_create(ctx._h, max_scans, max_points, ctypes.byref(h)), ctx, "graph")
         self._h = h.value
 
+    @staticmethod
+    def prepare(scans):
+        """The descriptor array of a frame, for set_scans (build it once for a
+        frame that is loaded many times)."""
+        return _scan_descs(scans)
+
     def set_scans(self, scans):
-        arr, self._keep = _scan_descs(scans)
-        _check(lib().bxg_graph_set_scans(self._h, arr, len(scans)), self.ctx, "graph frame")
+        """Load a frame: a list of scans as for insert_batch, or the result of
+        prepare().  Scans still pending from the previous frame are not waited
+        for."""
+        if isinstance(scans, tuple):
+            arr, keep = scans
+        else:
+            arr, keep = _scan_descs(scans)
+        self._keep = keep
+        _check(lib().bxg_graph_set_scans(self._h, arr, len(arr)), self.ctx, "graph frame")
 
     def launch(self):
         _check(lib().bxg_graph_launch(self._h), self.ctx, "graph launch")

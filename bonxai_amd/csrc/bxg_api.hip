@@ -11,7 +11,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -36,6 +38,17 @@ struct PinnedSlot {
 struct Prof {
   const char* name;
   hipEvent_t a, b;
+};
+
+// An event marking the completion of counter snapshots, shared by every map
+// whose snapshot it covers (a scan wave's maps share one); the event goes
+// back to the context's pool when the last map lets go of it.
+struct SnapEvent {
+  hipEvent_t e = nullptr;
+  std::vector<hipEvent_t>* pool = nullptr;
+  ~SnapEvent() {
+    if (e) pool->push_back(e);
+  }
 };
 
 }  // namespace
@@ -98,6 +111,7 @@ struct bxg_ctx {
   uint32_t root_slack = 2;
   std::vector<Prof> prof_pending;
   std::vector<hipEvent_t> ev_free;
+  std::vector<hipEvent_t> snap_pool;  // free snapshot events (SnapEvent)
   std::map<std::string, std::pair<uint64_t, double>> prof_totals;
 
   std::vector<bxg_map*> maps;
@@ -109,7 +123,7 @@ struct bxg_map {
   MapDev* d = nullptr;
   MapCounters* ctr = nullptr;
   MapCounters* h_ctr = nullptr;  // pinned mirror of the counters
-  hipEvent_t ev_ctr = nullptr;   // completion of the last counter copy
+  std::shared_ptr<SnapEvent> snap;  // completion of the last snapshot into h_ctr
   bool ctr_pending = false;
   uint32_t root_cap = 0;
   uint32_t leaf_cap = 0;
@@ -124,7 +138,12 @@ struct bxg_map {
   // asynchronous inserts enqueued since the map was last confirmed, and the
   // device's scans_applied before the first of them (see settle)
   std::vector<bxg_scan_desc> pending;
+  // the map's own copies of their sensor-to-world matrices (the caller's may
+  // be gone by the time an overflowed insert runs again); a deque keeps the
+  // copies in place as more are added
+  std::deque<std::array<float, 16>> pending_tfs;
   uint64_t pending_base = 0;
+  uint64_t replays = 0;  // pending inserts run again by settle after an overflow
 };
 
 struct bxg_graph {
@@ -340,7 +359,27 @@ int read_counters(bxg_map* m) {
   BXG_HIP(c, hipMemcpyAsync(m->h_ctr, m->ctr, sizeof(MapCounters), hipMemcpyDeviceToHost, c->stream));
   BXG_TRY(ctx_sync(c));
   m->ctr_pending = false;
+  m->snap.reset();
   take_counters(m);
+  return BXG_OK;
+}
+
+// Mark the snapshots enqueued so far for maps[0..n) with one event on the
+// context stream (polled lazily by poll_counters).
+int snapshot_event(bxg_ctx* c, bxg_map* const* maps, size_t n) {
+  auto ev = std::make_shared<SnapEvent>();
+  ev->pool = &c->snap_pool;
+  if (!c->snap_pool.empty()) {
+    ev->e = c->snap_pool.back();
+    c->snap_pool.pop_back();
+  } else {
+    BXG_HIP(c, hipEventCreateWithFlags(&ev->e, hipEventDisableTiming));
+  }
+  BXG_HIP(c, hipEventRecord(ev->e, c->stream));
+  for (size_t i = 0; i < n; ++i) {
+    maps[i]->snap = ev;
+    maps[i]->ctr_pending = true;
+  }
   return BXG_OK;
 }
 
@@ -348,15 +387,14 @@ int read_counters(bxg_map* m) {
 int snapshot_counters(bxg_map* m) {
   bxg_ctx* c = m->ctx;
   BXG_HIP(c, hipMemcpyAsync(m->h_ctr, m->ctr, sizeof(MapCounters), hipMemcpyDeviceToHost, c->stream));
-  BXG_HIP(c, hipEventRecord(m->ev_ctr, c->stream));
-  m->ctr_pending = true;
-  return BXG_OK;
+  return snapshot_event(c, &m, 1);
 }
 
 // take a finished snapshot into account without blocking
 void poll_counters(bxg_map* m) {
-  if (m->ctr_pending && hipEventQuery(m->ev_ctr) == hipSuccess) {
+  if (m->ctr_pending && hipEventQuery(m->snap->e) == hipSuccess) {
     m->ctr_pending = false;
+    m->snap.reset();
     take_counters(m);
   }
 }
@@ -676,6 +714,9 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
                           : std::max<uint32_t>(1, std::min<uint32_t>(cus * c->small_ray_blocks_per_cu, most8));
   const uint32_t gl = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->leaf_blocks_per_cu, most));
   sc.chunk_max = c->ray_chunk;
+  // new-leaf mask pass: a wavefront per new leaf, grid-stride over them
+  const uint32_t gm = std::max<uint32_t>(1, std::min<uint32_t>(cus * 8, gp));
+  BXG_TRY(launch(c, "k_scan_begin", k_scan_begin, 1, kBlock, hdr, desc));
   BXG_TRY(launch(c, "k_scan_classify", k_scan_classify, gp, kBlock, hdr, desc, sc));
   BXG_TRY(launch(c, "k_scan_winners", k_scan_winners, gp, kBlock, hdr, desc, sc));
   BXG_TRY(launch(c, "k_scan_bucket_reduce", k_scan_bucket_reduce, kTiles, 1024, hdr, sc));
@@ -689,11 +730,14 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
     BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<true>, gr, rb, desc, sc));
     if (any_extra)
       BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<true>, nscans, kBlock, hdr, desc));
+    BXG_TRY(launch(c, "k_scan_new_endpoints", k_scan_new_endpoints<true>, gw, kBlock, desc, sc));
   } else {
     BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<false>, gr, rb, desc, sc));
     if (any_extra)
       BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<false>, nscans, kBlock, hdr, desc));
+    BXG_TRY(launch(c, "k_scan_new_endpoints", k_scan_new_endpoints<false>, gw, kBlock, desc, sc));
   }
+  BXG_TRY(launch(c, "k_scan_new_masks", k_scan_new_masks, gm, kBlock, hdr, desc));
   BXG_TRY(launch(c, "k_scan_finalize", k_scan_finalize, 1, kBlock, hdr, desc, sc));
   return BXG_OK;
 }
@@ -738,8 +782,10 @@ int run_wave(bxg_ctx* c, const bxg_scan_desc* scans, uint32_t nscans, uint32_t e
   BXG_TRY(upload(c, c->d_desc, descs.data(), descs.size() * sizeof(ScanDesc)));
   BXG_TRY(upload(c, c->d_hdr, &hdr, sizeof(hdr)));
   BXG_TRY(enqueue_scan_kernels(c, c->d_hdr, c->d_desc, c->sc, total, nscans, any_extra));
-  for (uint32_t s = 0; s < nscans; ++s) BXG_TRY(snapshot_counters(scans[s].map));
-  return BXG_OK;
+  // k_scan_finalize wrote every map's counters to its host mirror
+  std::vector<bxg_map*> maps(nscans);
+  for (uint32_t s = 0; s < nscans; ++s) maps[s] = scans[s].map;
+  return snapshot_event(c, maps.data(), maps.size());
 }
 
 // split a batch into waves of distinct maps, keeping input order per map
@@ -781,6 +827,7 @@ int clear_status(bxg_map* m) {
   BXG_TRY(ctx_sync(m->ctx));
   // any snapshot enqueued before has landed; the mirror follows the device
   m->ctr_pending = false;
+  m->snap.reset();
   m->h_ctr->status = 0;
   return BXG_OK;
 }
@@ -796,6 +843,17 @@ int grow_after_overflow(bxg_map* m) {
 }
 
 // ---------------------------------------------------------- bulk ops
+
+// Remember an enqueued insert until the map is confirmed, with a copy of
+// its transform owned by the map.
+void pending_push(bxg_map* m, const bxg_scan_desc& s) {
+  m->pending.push_back(s);
+  if (s.sensor_to_world) {
+    m->pending_tfs.emplace_back();
+    std::memcpy(m->pending_tfs.back().data(), s.sensor_to_world, sizeof(float) * 16);
+    m->pending.back().sensor_to_world = m->pending_tfs.back().data();
+  }
+}
 
 // Start a pending window on m if none is open: the device's scans_applied
 // before the scans about to be enqueued, from the counter mirror when no
@@ -822,6 +880,8 @@ int settle(bxg_map* m) {
   bxg_ctx* c = m->ctx;
   std::vector<bxg_scan_desc> todo;
   todo.swap(m->pending);
+  std::deque<std::array<float, 16>> tfs;  // what todo's transforms point at
+  tfs.swap(m->pending_tfs);
   BXG_TRY(read_counters(m));
   const uint32_t overflow = kStOutOfPool | kStRootFull;
   if ((m->h_ctr->status & ~overflow) || !(m->h_ctr->status & overflow)) return BXG_OK;
@@ -832,6 +892,7 @@ int settle(bxg_map* m) {
         if (attempt > 8) return fail(c, BXG_ERR_OUT_OF_POOL, "could not grow the node pool");
         BXG_TRY(grow_after_overflow(m));
       }
+      ++m->replays;
       BXG_TRY(run_batch(c, &todo[i], 1, 0));
       BXG_TRY(read_counters(m));
       if (m->h_ctr->status & ~overflow) return BXG_OK;
@@ -978,6 +1039,7 @@ int bxg_ctx_destroy(bxg_ctx* c) {
     hipEventDestroy(p.b);
   }
   for (auto e : c->ev_free) hipEventDestroy(e);
+  for (auto e : c->snap_pool) hipEventDestroy(e);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return BXG_OK;
@@ -1201,7 +1263,7 @@ int bxg_map_create(bxg_ctx* c, double voxel_size, uint32_t inner_bits, uint32_t 
   m->ctr = (MapCounters*)p;
   if (rc == BXG_OK && hipHostMalloc((void**)&m->h_ctr, sizeof(MapCounters), hipHostMallocDefault) != hipSuccess)
     rc = BXG_ERR_HIP;
-  if (rc == BXG_OK && hipEventCreateWithFlags(&m->ev_ctr, hipEventDisableTiming) != hipSuccess)
+  if (rc == BXG_OK && hipHostGetDevicePointer((void**)&h.ctr_host, m->h_ctr, 0) != hipSuccess)
     rc = BXG_ERR_HIP;
   if (rc == BXG_OK) {
     std::memset(m->h_ctr, 0, sizeof(MapCounters));
@@ -1235,8 +1297,8 @@ int bxg_map_destroy(bxg_map* m) {
   dfree(m->d);
   dfree(m->ctr);
   dfree(m->d_extra);
+  m->snap.reset();
   if (m->h_ctr) hipHostFree(m->h_ctr);
-  if (m->ev_ctr) hipEventDestroy(m->ev_ctr);
   c->used -= std::min(c->used, m->bytes);
   c->maps.erase(std::remove(c->maps.begin(), c->maps.end(), m), c->maps.end());
   delete m;
@@ -1319,6 +1381,7 @@ int bxg_map_get_stats(bxg_map* m, bxg_map_stats* out) {
   out->last_rays = k.stat_rays;
   out->last_ray_updates = k.stat_ray_updates;
   out->last_ray_visits = k.stat_ray_visits;
+  out->replays = m->replays;
   out->update_count = (uint32_t)k.cur;
   out->status = k.status;
   return BXG_OK;
@@ -1494,9 +1557,9 @@ int bxg_export(bxg_map* m, int filter, int32_t* coords, void* vals, uint64_t cap
   BXG_HIP(c, hipMemsetAsync(c->d_small, 0, 8, c->stream));
   BXG_TRY(launch(c, "k_export", k_export, nblocks((uint64_t)nl * m->h.leaf_size), kBlock,
                  (const MapDev*)m->d, nl, (uint32_t)filter, coords, (uint8_t*)vals, cap,
-                 (uint32_t*)c->d_small));
+                 c->d_small));
   BXG_TRY(d2h_sync(c, c->h_small, c->d_small, 8));
-  *count = (uint32_t)c->h_small[0];
+  *count = c->h_small[0];
   return *count > cap ? fail(c, BXG_ERR_CAPACITY, "export buffer too small") : BXG_OK;
 }
 
@@ -1740,7 +1803,7 @@ int insert_async(bxg_map* m, const bxg_scan_desc& s) {
   }
   BXG_TRY(pending_begin(m));
   BXG_TRY(run_batch(c, &s, 1, 0));
-  m->pending.push_back(s);
+  pending_push(m, s);
   if (m->pending.size() >= kMaxPendingInserts) BXG_TRY(settle(m));
   return BXG_OK;
 }
@@ -1969,9 +2032,9 @@ int bxg_graph_set_scans(bxg_graph* g, const bxg_scan_desc* scans, uint32_t nscan
   bxg_ctx* c = g->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   if (nscans > g->max_scans) return fail(c, BXG_ERR_INVALID_ARG, "more scans than the graph holds");
-  // the previous frame's scans may still be pending on their maps, pointing
-  // at this graph's copies of their transforms: confirm them first
-  for (bxg_map* m : g->maps) BXG_TRY(settle(m));
+  // the previous frame's scans may still be pending on their maps; they
+  // hold their own copies of their transforms, so the frame can change
+  // without waiting for them (the descriptor upload is stream-ordered)
   std::vector<ScanDesc> descs;
   uint64_t total = 0;
   BXG_TRY(build_descs(c, scans, nscans, 0u, descs, &total));
@@ -2016,12 +2079,13 @@ int bxg_graph_launch(bxg_graph* g) {
   for (size_t s = 0; s < g->maps.size(); ++s)
     BXG_TRY(ensure_room(g->maps[s], scan_leaf_estimate(g->maps[s], g->npoints[s], g->ranges[s]), 1024));
   BXG_HIP(c, hipGraphLaunch(g->exec, c->stream));
+  // the frame's k_scan_finalize wrote each map's counters to its mirror
+  BXG_TRY(snapshot_event(c, g->maps.data(), g->maps.size()));
   // each map keeps its scan of the frame until it is confirmed (settle),
   // which re-runs it should the frame have overflowed the map's pools
   for (size_t s = 0; s < g->maps.size(); ++s) {
     bxg_map* m = g->maps[s];
-    BXG_TRY(snapshot_counters(m));
-    m->pending.push_back(g->scans[s]);
+    pending_push(m, g->scans[s]);
     if (m->pending.size() >= kMaxPendingInserts) BXG_TRY(settle(m));
   }
   return BXG_OK;
@@ -2031,8 +2095,7 @@ int bxg_graph_destroy(bxg_graph* g) {
   if (!g) return BXG_ERR_INVALID_ARG;
   bxg_ctx* c = g->ctx;
   std::lock_guard<std::recursive_mutex> lk(c->mu);
-  for (bxg_map* m : g->maps) settle(m);  // pending scans point at this graph's transforms
-  hipStreamSynchronize(c->stream);
+  hipStreamSynchronize(c->stream);  // the scratch below may still be in use
   if (g->exec) hipGraphExecDestroy(g->exec);
   if (g->graph) hipGraphDestroy(g->graph);
   dfree(g->sc.rec);

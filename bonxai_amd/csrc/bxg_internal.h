@@ -56,8 +56,12 @@ struct alignas(64) MapCounters {
   uint64_t stat_rays;
   uint64_t stat_ray_updates;
   uint64_t stat_ray_visits;  // counted runs: cells the scan's rays crossed
-  uint64_t pad;
+  // leaf_count when the running scan started: leaves at or above it were
+  // created by that scan, so all their cells were off (zero) before it
+  uint32_t leaf_base;
+  uint32_t pad;
 };
+static_assert(sizeof(MapCounters) == 64, "MapCounters is one cache line");
 
 struct alignas(64) MapDev {
   double inv_res;
@@ -74,6 +78,7 @@ struct alignas(64) MapDev {
   uint32_t leaf_cap;
   int32_t opt_miss, opt_hit, opt_cmin, opt_cmax, opt_thr;
   MapCounters* ctr;
+  MapCounters* ctr_host;  // pinned host mirror, written at the end of every scan
 };
 
 // one scan of a (possibly batched) insertPointCloud call

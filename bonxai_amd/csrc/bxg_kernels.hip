@@ -100,6 +100,39 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
   return base + (uint32_t)__popcll(b & lanes_below());
 }
 
+// 64-bit form of wave_append, for counts that may pass 2^32.
+__device__ __forceinline__ uint64_t wave_append64(uint64_t* counter, bool pred) {
+  const uint64_t b = __ballot(pred);
+  if (b == 0) return 0;
+  const int leader = __ffsll((unsigned long long)b) - 1;
+  uint64_t base = 0;
+  if ((int)lane_id() == leader) base = add64(counter, (uint64_t)__popcll(b));
+  base = __shfl(base, leader);
+  return base + (uint64_t)__popcll(b & lanes_below());
+}
+
+// Bump allocation from per-map counters, aggregated over a wavefront: the
+// active lanes with pred set are grouped by counter address (lanes of one
+// wavefront usually serve one map), each group takes one atomic add of its
+// size, and each lane gets base + its rank in the group.  Every active lane
+// must call it (it is a wavefront collective); the result is meaningful only
+// where pred is set.
+__device__ __forceinline__ uint32_t wave_bump(uint32_t* counter, bool pred) {
+  uint64_t pending = __ballot(pred);
+  uint32_t mine = 0;
+  while (pending) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const unsigned long long lc = __shfl((unsigned long long)counter, leader);
+    const uint64_t same = __ballot(pred && (unsigned long long)counter == lc);
+    uint32_t base = 0;
+    if ((int)lane_id() == leader) base = add32((uint32_t*)lc, (uint32_t)__popcll(same));
+    base = __shfl(base, leader);
+    if ((same >> lane_id()) & 1ull) mine = base + (uint32_t)__popcll(same & lanes_below());
+    pending &= ~same;
+  }
+  return mine;
+}
+
 // ------------------------------------------------------------ map view
 
 // A register copy of a map header with global-address-space pointers.
@@ -159,22 +192,31 @@ __device__ __forceinline__ uint32_t root_hash(uint64_t key, uint32_t mask) {
 
 // Find-or-insert of a root key (lock-free CAS insert, linear probing).
 // Returns the slot, or -1 (and sets kStRootFull) after max_probe probes.
+// The roots a wavefront inserts are counted with one atomic per map
+// (wave_bump), so every active lane must call it.
 __device__ int32_t root_insert(const MapView& m, uint64_t key) {
   uint32_t h = root_hash(key, m.root_mask);
+  int32_t slot = -1;
+  bool inserted = false;
   for (uint32_t p = 0; p < m.max_probe; ++p) {
     uint64_t k = m.root_keys[h];
     if (k == kEmptyKey) {
       k = cas64((uint64_t*)&m.root_keys[h], kEmptyKey, key);
       if (k == kEmptyKey) {
-        add32(&m.ctr->root_count, 1u);
-        return (int32_t)h;
+        inserted = true;
+        slot = (int32_t)h;
+        break;
       }
     }
-    if (k == key) return (int32_t)h;
+    if (k == key) {
+      slot = (int32_t)h;
+      break;
+    }
     h = (h + 1) & m.root_mask;
   }
-  or32(&m.ctr->status, kStRootFull);
-  return -1;
+  wave_bump(&m.ctr->root_count, inserted);
+  if (slot < 0) or32(&m.ctr->status, kStRootFull);
+  return slot;
 }
 
 // Lookup of a root key in a phase where no insert runs concurrently.
@@ -189,18 +231,28 @@ __device__ __forceinline__ int32_t root_find(const MapView& m, uint64_t key) {
   return -1;
 }
 
-// Make sure leaf `ci` of root `slot` exists.  The lane that moves the child
-// word from EMPTY to PENDING allocates the leaf and publishes its index; all
-// others return at once (they do not need the index before the next kernel).
-// Returns the child word as seen (a leaf index, PENDING, or EMPTY on failure).
-__device__ uint32_t leaf_request(const MapView& m, uint32_t slot, uint32_t ci, int32_t lx,
-                                 int32_t ly, int32_t lz) {
-  uint32_t* cp = (uint32_t*)&m.children[(uint64_t)slot * m.isz + ci];
+// Claim of the inner-grid entry *cp for a new leaf: true if this lane moved
+// it from EMPTY to PENDING; otherwise `seen` is the entry as found (a leaf
+// index, or PENDING while another lane creates the leaf).
+__device__ __forceinline__ bool leaf_claim(uint32_t* cp, uint32_t& seen) {
   uint32_t v = *gp(cp);
-  if (v != kEmptyChild) return v;
-  v = cas32(cp, kEmptyChild, kPendingChild);
-  if (v != kEmptyChild) return v;
-  const uint32_t idx = add32(&m.ctr->leaf_count, 1u);
+  if (v == kEmptyChild) v = cas32(cp, kEmptyChild, kPendingChild);
+  seen = v;
+  return v == kEmptyChild;
+}
+
+// Leaf indices for the lanes that claimed an entry (allocateLeafGrid and
+// GridBlockAllocator::allocateBlock, bonxai.hpp:572-586 and
+// grid_allocator.hpp:94-123): one wavefront-aggregated bump of the map's
+// leaf counter per wavefront and map, then each claimer writes its leaf's
+// origin and publishes the index in the entry.  A claimer past the pool's
+// capacity puts the entry back to EMPTY and flags the map (the host grows the
+// pool and runs the call again).  Every active lane must call it; returns the
+// new index, or EMPTY on failure, where `claimed` is set.
+__device__ __forceinline__ uint32_t leaf_publish(const MapView& m, bool claimed, uint32_t* cp,
+                                                 uint32_t slot, int32_t lx, int32_t ly, int32_t lz) {
+  const uint32_t idx = wave_bump(&m.ctr->leaf_count, claimed);
+  if (!claimed) return kEmptyChild;
   if (idx >= m.leaf_cap) {
     st32(cp, kEmptyChild);
     or32(&m.ctr->status, kStOutOfPool);
@@ -209,6 +261,18 @@ __device__ uint32_t leaf_request(const MapView& m, uint32_t slot, uint32_t ci, i
   m.origin[idx] = make_int4(lx, ly, lz, (int32_t)slot);
   st32(cp, idx);
   return idx;
+}
+
+// Make sure leaf `ci` of root `slot` exists (a wavefront collective, through
+// leaf_publish).  Returns the child word as seen: a leaf index, PENDING while
+// another lane creates it, or EMPTY when the pool ran out.
+__device__ __forceinline__ uint32_t leaf_request(const MapView& m, uint32_t slot, uint32_t ci,
+                                                 int32_t lx, int32_t ly, int32_t lz) {
+  uint32_t* cp = (uint32_t*)&m.children[(uint64_t)slot * m.isz + ci];
+  uint32_t seen;
+  const bool won = leaf_claim(cp, seen);
+  const uint32_t idx = leaf_publish(m, won, cp, slot, lx, ly, lz);
+  return won ? idx : seen;
 }
 
 // leaf index holding coordinate c, or kNone (read-only phases)
@@ -467,7 +531,7 @@ constexpr int kLeafRun = 2;
 __device__ __forceinline__ void leaves_step(const MapView& m, LeafWalk& lw, RootCache& rc) {
   const int32_t slot = root_slot<true>(m, rc, lw.lx, lw.ly, lw.lz);
   const int32_t rmask = ~((1 << m.log2n) - 1);
-  int32_t lx[kLeafRun], ly[kLeafRun], lz[kLeafRun];
+  int32_t lx[kLeafRun] = {}, ly[kLeafRun] = {}, lz[kLeafRun] = {};
   int n = 0;
 #pragma unroll
   for (int j = 0; j < kLeafRun; ++j) {
@@ -482,14 +546,22 @@ __device__ __forceinline__ void leaves_step(const MapView& m, LeafWalk& lw, Root
   }
   if (slot < 0) return;  // root directory full: the call fails
   uint32_t v[kLeafRun];
+  uint32_t* cp[kLeafRun];
 #pragma unroll
-  for (int j = 0; j < kLeafRun; ++j)
-    v[j] = j < n ? m.children[(uint64_t)slot * m.isz + inner_index(lx[j], ly[j], lz[j], m.lb, m.ib)]
-                 : 0u;
+  for (int j = 0; j < kLeafRun; ++j) {
+    cp[j] = (uint32_t*)&m.children[(uint64_t)slot * m.isz + inner_index(lx[j], ly[j], lz[j], m.lb, m.ib)];
+    v[j] = j < n ? *gp(cp[j]) : 0u;
+  }
+  // claims first, then one aggregated allocation per run slot for the
+  // whole wavefront (every active lane takes part, claimed or not)
+  bool won[kLeafRun];
 #pragma unroll
-  for (int j = 0; j < kLeafRun; ++j)
-    if (j < n && v[j] == kEmptyChild)
-      leaf_request(m, (uint32_t)slot, inner_index(lx[j], ly[j], lz[j], m.lb, m.ib), lx[j], ly[j], lz[j]);
+  for (int j = 0; j < kLeafRun; ++j) {
+    uint32_t seen;
+    won[j] = (j < n && v[j] == kEmptyChild) ? leaf_claim(cp[j], seen) : false;
+  }
+#pragma unroll
+  for (int j = 0; j < kLeafRun; ++j) leaf_publish(m, won[j], cp[j], (uint32_t)slot, lx[j], ly[j], lz[j]);
 }
 
 // One leaf of a ray, update pass: the cells of the ray in the current leaf
@@ -507,15 +579,39 @@ __device__ __forceinline__ uint32_t walk_leaf(const MapView& m, const RayWalk& w
 // `leaf` is the leaf of the walk's current cell on entry; on return it is the
 // leaf of the next run (looked up before this run's cells are loaded, so the
 // two memory round trips overlap).
+//
+// Leaves at or above `lbase` were created by the running scan, so each of
+// their cells held zero (off) before it.  The ray update of such a cell gives
+// the same word whichever ray gets there first, cell_miss(0), and the only
+// cells of these leaves that must not end with it are endpoints of the scan,
+// which k_scan_new_endpoints writes after this pass.  So the runs in new
+// leaves are stored without being read, and their mask words are built after
+// the pass from the nonzero cells (k_scan_new_masks), with no atomic a cell.
+// In older leaves each cell is read, updated unless its update_id is cur, and
+// a cell turned on sets its mask bit with an atomicOr.
 template <bool kCount>
 __device__ __forceinline__ uint32_t apply_step(const MapView& m, RayWalk& w, RootCache& rc,
-                                               int32_t cur, uint32_t& next_leaf) {
+                                               int32_t cur, uint32_t lbase, uint32_t& next_leaf) {
   const uint32_t leaf = next_leaf;
   uint32_t li[kRun];
   const uint32_t n = collect_run(m, w, li);
   next_leaf = w.done() ? kNone : walk_leaf(m, w, rc);
   if (leaf == kNone) return 0;
   uint32_t BXG_G* base = (uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz;
+  if (leaf >= lbase) {
+    const uint32_t fresh = cell_miss(0u, m.miss, m.cmin, cur);
+    uint32_t nupd = 0;
+#pragma unroll
+    for (int j = 0; j < kRun; ++j) {
+      if ((uint32_t)j < n) {
+        if (kCount)
+          nupd += cas32((uint32_t*)&base[li[j]], 0u, fresh) == 0u ? 1u : 0u;
+        else
+          base[li[j]] = fresh;
+      }
+    }
+    return nupd;
+  }
   uint32_t v[kRun];
 #pragma unroll
   for (int j = 0; j < kRun; ++j)
@@ -826,8 +922,8 @@ __global__ void __launch_bounds__(256) k_scan_scatter(const BatchHdr* __restrict
 // `first` tells the caller the unit holds the ray's cell 0.
 __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const ScanScratch& sc,
                                           uint32_t unit, uint32_t lg, MapView& m, RayWalk& w,
-                                          RootCache& rc, int32_t& cur, bool check_status,
-                                          bool& first) {
+                                          RootCache& rc, int32_t& cur, uint32_t& lbase,
+                                          bool check_status, bool& first) {
   const uint32_t idx = unit >> lg, seg = unit & ((1u << lg) - 1u);
   first = seg == 0;
   const uint32_t gi = sc.winners[idx];
@@ -838,6 +934,7 @@ __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const
   m = load_view(d.map);
   if (check_status && (map_status(m) & kFatal)) return false;
   cur = map_cur(m);
+  lbase = *gp(&m.ctr->leaf_base);
   w.init(d.ox, d.oy, d.oz, r.x, r.y, r.z);
   if (lg) {
     const int64_t n = 1ll << lg;
@@ -904,6 +1001,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
   LeafWalk lw;  // the leaf pass walks leaves, not cells
   RootCache rc;
   int32_t cur = 0;
+  uint32_t lbase = 0;
   uint32_t nupd = 0;
   uint32_t next_leaf = kNone;
   // counted form: per-lane totals, flushed when the lane's map changes
@@ -943,7 +1041,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       const uint32_t take = want < avail ? want : avail;
       const uint32_t rank = (uint32_t)__popcll(need & lanes_below());
       if (!have && rank < take) {
-        have = start_ray(sd, sc, chunk_pos + rank, lg, m, w, rc, cur, kMode != 0, first);
+        have = start_ray(sd, sc, chunk_pos + rank, lg, m, w, rc, cur, lbase, kMode != 0, first);
         nupd = 0;
         if (kMode == 0 && have) lw.init(w, m.lb);
         if (kMode != 0 && have) next_leaf = walk_leaf(m, w, rc);
@@ -987,7 +1085,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       if (kMode == 0) {
         leaves_step(m, lw, rc);
       } else {
-        nupd += apply_step<kMode == 2>(m, w, rc, cur, next_leaf);
+        nupd += apply_step<kMode == 2>(m, w, rc, cur, lbase, next_leaf);
       }
       if (kMode == 0 ? lw.done() : w.done()) {
         have = false;
@@ -1034,7 +1132,10 @@ __global__ void __launch_bounds__(256) k_scan_extra_leaves(const BatchHdr* __res
 }
 
 // Phase 3 (per winner): endpoint hit/miss update (probabilistic_map.cpp:30-54)
-// and release of the endpoint-table slot.  One writer per endpoint cell.
+// of the endpoints in leaves that existed before the scan, and release of
+// the endpoint-table slot.  One writer per endpoint cell.  An endpoint in a
+// leaf the scan created is left for k_scan_new_endpoints (after the ray
+// pass): its leaf index goes to tpos.
 __global__ void __launch_bounds__(256) k_scan_endpoints(const ScanDesc* __restrict__ sd,
                                                         ScanScratch sc) {
   const uint32_t nwin = sc.counters[0];
@@ -1056,7 +1157,9 @@ __global__ void __launch_bounds__(256) k_scan_endpoints(const ScanDesc* __restri
       const ScanDesc& d = sd[(uint32_t)r.w >> 16];
       const MapView m = load_view(d.map);
       const uint32_t leaf = (map_status(m) & kFatal) ? kNone : find_leaf(m, r.x, r.y, r.z);
-      if (leaf != kNone) {
+      const bool fresh = leaf != kNone && leaf >= *gp(&m.ctr->leaf_base);
+      sc.tpos[gi] = fresh ? leaf : kNone;
+      if (leaf != kNone && !fresh) {
         const uint32_t li = leaf_index(r.x, r.y, r.z, m.lb);
         uint32_t BXG_G* cell = (uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz + li;
         const uint32_t w = *cell;
@@ -1074,6 +1177,54 @@ __global__ void __launch_bounds__(256) k_scan_endpoints(const ScanDesc* __restri
       MapCounters* lc = (MapCounters*)__shfl((unsigned long long)ctr, leader);
       const uint64_t same = __ballot(updated && ctr == lc);
       if ((int)lane_id() == leader) add64(&lc->stat_endpoint_updates, (uint64_t)__popcll(same));
+      pending &= ~same;
+    }
+  }
+}
+
+// Phase 4d (per winner): the endpoint update of the endpoints in leaves the
+// scan created (k_scan_endpoints left their leaf index in tpos).  Such a
+// cell held zero before the scan, so its endpoint word is cell_hit(0) or
+// cell_miss(0), whatever the ray pass stored there meanwhile (the reference
+// updates endpoints first and its rays skip them, probabilistic_map.cpp:30-54,
+// 81-89).  The counted form takes back the ray update the ray pass counted
+// for a cell that turns out to be an endpoint.
+template <bool kCount>
+__global__ void __launch_bounds__(256) k_scan_new_endpoints(const ScanDesc* __restrict__ sd,
+                                                            ScanScratch sc) {
+  const uint32_t nwin = sc.counters[0];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t start = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t base = start - (start & 63u); base < nwin; base += stride) {
+    const uint32_t idx = base + (start & 63u);
+    uint32_t updated = 0, retracted = 0;
+    MapCounters* ctr = nullptr;
+    if (idx < nwin) {
+      const uint32_t gi = sc.winners[idx];
+      const uint32_t leaf = sc.tpos[gi];
+      if (leaf != kNone) {
+        const int4 r = sc.rec[gi];
+        const MapView m = load_view(sd[(uint32_t)r.w >> 16].map);
+        const int32_t cur = map_cur(m);
+        uint32_t BXG_G* cell =
+            (uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz + leaf_index(r.x, r.y, r.z, m.lb);
+        const uint32_t word = (r.w & kRecHit) ? cell_hit(0u, m.hit, m.cmax, cur) : cell_miss(0u, m.miss, m.cmin, cur);
+        if (kCount) retracted = *cell != 0u ? 1u : 0u;
+        *cell = word;
+        updated = 1;
+        ctr = m.ctr;
+      }
+    }
+    uint64_t pending = __ballot(updated);
+    while (pending) {
+      const int leader = __ffsll((unsigned long long)pending) - 1;
+      MapCounters* lc = (MapCounters*)__shfl((unsigned long long)ctr, leader);
+      const uint64_t same = __ballot(updated && ctr == lc);
+      const uint64_t back = __ballot(retracted && ctr == lc);
+      if ((int)lane_id() == leader) {
+        add64(&lc->stat_endpoint_updates, (uint64_t)__popcll(same));
+        if (kCount && back) add64(&lc->stat_ray_updates, (uint64_t)0 - (uint64_t)__popcll(back));
+      }
       pending &= ~same;
     }
   }
@@ -1098,6 +1249,7 @@ __global__ void __launch_bounds__(256) k_scan_extra_rays(const BatchHdr* __restr
   const MapView m = load_view(d.map);
   if (map_status(m) & kFatal) return;
   const int32_t cur = map_cur(m);
+  const uint32_t lbase = *gp(&m.ctr->leaf_base);
   for (uint32_t j = threadIdx.x; j < d.nextra; j += blockDim.x) {
     const int4 e = d.extra[j];
     if (e.x == d.ox && e.y == d.oy && e.z == d.oz) continue;
@@ -1107,7 +1259,7 @@ __global__ void __launch_bounds__(256) k_scan_extra_rays(const BatchHdr* __restr
     w.init(d.ox, d.oy, d.oz, e.x, e.y, e.z);
     uint32_t n = 0;
     uint32_t next_leaf = walk_leaf(m, w, rc);
-    while (!w.done()) n += apply_step<kCount>(m, w, rc, cur, next_leaf);
+    while (!w.done()) n += apply_step<kCount>(m, w, rc, cur, lbase, next_leaf);
     if (kCount) {
       add64(&m.ctr->stat_ray_updates, n);
       add64(&m.ctr->stat_rays, 1);
@@ -1116,17 +1268,84 @@ __global__ void __launch_bounds__(256) k_scan_extra_rays(const BatchHdr* __restr
   }
 }
 
+// Phase 4c: the mask words of the leaves the scan created.  Their cells were
+// all zero (off) before it, and every cell the scan updated holds a nonzero
+// word (update_id = cur, 1..3), so such a leaf's mask is exactly its nonzero
+// cells: one ballot per 64 cells, in place of an atomicOr per cell turned on
+// (Accessor::value(create) sets the bit, bonxai.hpp:485-490).  One wavefront
+// per new leaf; the new leaves of each scan's map are [leaf_base,
+// leaf_count).  Maps whose call failed wrote no cell and are skipped.
+__global__ void __launch_bounds__(256) k_scan_new_masks(const BatchHdr* __restrict__ hdr,
+                                                        const ScanDesc* __restrict__ sd) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+  const uint32_t nscans = hdr->nscans;
+  for (uint32_t s = 0; s < nscans; ++s) {
+    const MapView m = load_view(sd[s].map);
+    const MapCounters BXG_G* ctr = gp(m.ctr);
+    if (ctr->status & kFatal) continue;
+    const uint32_t lo = ctr->leaf_base;
+    const uint32_t hi = min(ctr->leaf_count, m.leaf_cap);
+    if (m.lw == 8) {  // 8^3 leaves (the default leaf_bits 3): eight loads in flight
+      for (uint32_t leaf = lo + wave; leaf < hi; leaf += nwaves) {
+        const uint32_t BXG_G* cells = (const uint32_t BXG_G*)m.cells + (uint64_t)leaf * 512u;
+        uint32_t v[8];
+#pragma unroll
+        for (uint32_t w = 0; w < 8; ++w) v[w] = cells[w * 64u + lane];
+        uint64_t b[8];
+#pragma unroll
+        for (uint32_t w = 0; w < 8; ++w) b[w] = __ballot(v[w] != 0u);
+        // lane w stores word w: one 64-byte write per leaf
+        uint64_t mine = b[0];
+#pragma unroll
+        for (uint32_t w = 1; w < 8; ++w) mine = lane == w ? b[w] : mine;
+        if (lane < 8) m.masks[(uint64_t)leaf * 8u + lane] = mine;
+      }
+    } else {
+      for (uint32_t leaf = lo + wave; leaf < hi; leaf += nwaves) {
+        const uint32_t BXG_G* cells = (const uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz;
+        for (uint32_t w = 0; w < m.lw; ++w) {
+          const uint32_t i = w * 64u + lane;
+          const uint64_t b = __ballot((i < m.lsz ? cells[i] : 0u) != 0u);
+          if (lane == 0) m.masks[(uint64_t)leaf * m.lw + w] = b;
+        }
+      }
+    }
+  }
+}
+
+// Phase 0 (per scan): the leaf count at the start of the scan, so that later
+// phases can tell the leaves the scan creates from older ones.
+__global__ void k_scan_begin(const BatchHdr* __restrict__ hdr, const ScanDesc* __restrict__ sd) {
+  for (uint32_t s = threadIdx.x; s < hdr->nscans; s += blockDim.x) {
+    const MapView m = load_view(sd[s].map);
+    MapCounters BXG_G* ctr = gp(m.ctr);
+    ctr->leaf_base = min(ctr->leaf_count, m.leaf_cap);
+  }
+}
+
 // Phase 5 (per scan): advance update_count 1 -> 2 -> 3 -> 1
-// (probabilistic_map.cpp:103-105) unless the scan failed; reset the batch.
+// (probabilistic_map.cpp:103-105) unless the scan failed; reset the batch;
+// copy every scanned map's counters to its pinned host mirror, so the host
+// learns pool use and errors from one event after the call (no copy a map).
 __global__ void k_scan_finalize(const BatchHdr* __restrict__ hdr, const ScanDesc* __restrict__ sd,
                                 ScanScratch sc) {
   for (uint32_t s = threadIdx.x; s < hdr->nscans; s += blockDim.x) {
-    if (sd[s].flags & kScanNoRays) continue;
-    MapCounters BXG_G* ctr = gp(load_view(sd[s].map).ctr);
-    if (ctr->status & kFatal) continue;
-    const int32_t c = ctr->cur + 1;
-    ctr->cur = c == 4 ? 1 : c;
-    ctr->scans_applied += 1;
+    const MapDev BXG_G* md = (const MapDev BXG_G*)sd[s].map;
+    MapCounters BXG_G* ctr = gp(md->ctr);
+    if (!(sd[s].flags & kScanNoRays) && !(ctr->status & kFatal)) {
+      const int32_t c = ctr->cur + 1;
+      ctr->cur = c == 4 ? 1 : c;
+      ctr->scans_applied += 1;
+    }
+    const uint64_t BXG_G* src = (const uint64_t BXG_G*)ctr;
+    uint64_t* dst = (uint64_t*)md->ctr_host;
+    uint64_t v[sizeof(MapCounters) / 8];
+#pragma unroll
+    for (uint32_t k = 0; k < sizeof(MapCounters) / 8; ++k) v[k] = src[k];
+#pragma unroll
+    for (uint32_t k = 0; k < sizeof(MapCounters) / 8; ++k) dst[k] = v[k];
   }
   if (threadIdx.x <= 4 * kRegions) sc.counters[threadIdx.x] = 0;
 }
@@ -1414,7 +1633,7 @@ __global__ void __launch_bounds__(256) k_classify_cells(const MapDev* __restrict
 __global__ void __launch_bounds__(256) k_export(const MapDev* __restrict__ mp, uint32_t nleaves,
                                                 uint32_t filter, int32_t* __restrict__ coords,
                                                 uint8_t* __restrict__ vals, uint64_t cap,
-                                                uint32_t* __restrict__ count) {
+                                                uint64_t* __restrict__ count) {
   const MapView m = load_view(mp);
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t leaf = t >> (3 * m.lb);
@@ -1427,17 +1646,17 @@ __global__ void __launch_bounds__(256) k_export(const MapDev* __restrict__ mp, u
       pass = filter == 1 ? p > m.thr : p < m.thr;
     }
   }
-  const uint32_t pos = wave_append(count, pass);
+  const uint64_t pos = wave_append64(count, pass);
   if (!pass || pos >= cap) return;
   const int4 o = m.origin[leaf];
   const uint32_t lb = m.lb, lm = (1u << lb) - 1;
   if (coords) {
-    coords[3 * (uint64_t)pos] = o.x | (int32_t)(li & lm);
-    coords[3 * (uint64_t)pos + 1] = o.y | (int32_t)((li >> lb) & lm);
-    coords[3 * (uint64_t)pos + 2] = o.z | (int32_t)((li >> (2 * lb)) & lm);
+    coords[3 * pos] = o.x | (int32_t)(li & lm);
+    coords[3 * pos + 1] = o.y | (int32_t)((li >> lb) & lm);
+    coords[3 * pos + 2] = o.z | (int32_t)((li >> (2 * lb)) & lm);
   }
   const uint32_t vb = m.vb;
-  if (vals && vb) copy_bytes(gp(vals) + (uint64_t)pos * vb, m.cells + (leaf * m.lsz + li) * vb, vb);
+  if (vals && vb) copy_bytes(gp(vals) + pos * vb, m.cells + (leaf * m.lsz + li) * vb, vb);
 }
 
 // order-free digest of the on cells (count, sum, xor, sum of remix)

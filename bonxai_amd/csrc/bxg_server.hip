@@ -132,7 +132,7 @@ void srv_launch_read(bxg_server* s, SrvReq* r) {
                      r->n, (uint8_t*)r->vals, r->found);
     } else {
       // the map is idle: its last counter snapshot has landed or is about to
-      if (m->ctr_pending) hipEventSynchronize(m->ev_ctr);
+      if (m->ctr_pending) hipEventSynchronize(m->snap->e);
       poll_counters(m);
       const uint32_t nl = (uint32_t)m->known_leaves;
       uint64_t* dc = s->d_counts + r->slot;
@@ -140,7 +140,7 @@ void srv_launch_read(bxg_server* s, SrvReq* r) {
       if (rc == BXG_OK)
         rc = launch_on(c, st, "k_export", k_export, nblocks((uint64_t)nl * m->h.leaf_size), kBlock,
                        (const MapDev*)m->d, nl, (uint32_t)r->filter, r->out_coords, (uint8_t*)r->vals,
-                       r->cap, (uint32_t*)dc);
+                       r->cap, dc);
       if (rc == BXG_OK && hipMemcpyAsync(s->h_counts + r->slot, dc, 8, hipMemcpyDeviceToHost, st) != hipSuccess)
         rc = fail(c, BXG_ERR_HIP, "hipMemcpyAsync");
     }

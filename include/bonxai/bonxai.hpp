@@ -349,7 +349,16 @@ class VoxelGrid {
   /// through deviceMap(); `may_create` says whether leaves may have been added.
   void refresh(bool may_create) const {
     std::lock_guard<std::recursive_mutex> lk(st_->mu);
-    refreshLocked(may_create);
+    refreshLocked(may_create, nullptr, 0.0);
+  }
+
+  /// refresh(true) for a device-side change that touched no cell farther than
+  /// `radius` (world units) from `center`, such as a scan from `center` whose
+  /// rays end within that reach: only the cached pages whose leaf box comes
+  /// within `radius` of `center` are re-read.
+  void refreshNear(const Point3D& center, double radius) const {
+    std::lock_guard<std::recursive_mutex> lk(st_->mu);
+    refreshLocked(true, &center, radius);
   }
 
   /// Accessor::setValue over a batch, in input order, on the GPU.
@@ -532,7 +541,21 @@ class VoxelGrid {
     }
   }
 
-  void refreshLocked(bool may_create) const {
+  // whether the box of the leaf at `key` comes within `radius` of `c`
+  bool leafNear(const CoordT& key, const Point3D& c, double radius) const {
+    const double res = st_->res, side = double(1u << st_->leaf_bits) * res;
+    const double lo[3] = {double(key.x) * res, double(key.y) * res, double(key.z) * res};
+    const double p[3] = {c.x, c.y, c.z};
+    double d2 = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double below = lo[a] - p[a], above = p[a] - (lo[a] + side);
+      const double d = below > 0 ? below : (above > 0 ? above : 0.0);
+      d2 += d * d;
+    }
+    return !(d2 > radius * radius);  // NaN or infinite radius: near
+  }
+
+  void refreshLocked(bool may_create, const Point3D* center, double radius) const {
     State& s = *st_;
     if (may_create) s.complete = false;
     std::vector<CoordT> keys;
@@ -540,7 +563,7 @@ class VoxelGrid {
       if (!it->second) {
         it = s.pages.erase(it);
       } else {
-        keys.push_back(it->first);
+        if (!center || leafNear(it->first, *center, radius)) keys.push_back(it->first);
         ++it;
       }
     }

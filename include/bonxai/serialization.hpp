@@ -157,13 +157,15 @@ inline HeaderInfo GetHeaderInfo(std::string header) {
   return info;
 }
 
-/// Build a grid from the stream after its header line (serialization.hpp:159-199).
+/// Read the grid stream after its header line into `grid`, an empty grid of
+/// the header's type and shape (the body of Deserialize).
 template <typename DataT>
-inline VoxelGrid<DataT> Deserialize(std::istream& input, HeaderInfo info) {
+inline void DeserializeInto(std::istream& input, const HeaderInfo& info, VoxelGrid<DataT>& grid) {
   if (details::demangle(typeid(DataT).name()) != info.type_name)
     throw std::runtime_error("DataT does not match");
+  if (int(grid.innerBits()) != info.inner_bits || int(grid.leafBits()) != info.leaf_bits)
+    throw std::runtime_error("grid shape does not match the stream");
   constexpr size_t vb = std::is_same_v<DataT, EmptyVoxel> ? 0 : sizeof(DataT);
-  VoxelGrid<DataT> grid(info.resolution, uint8_t(info.inner_bits), uint8_t(info.leaf_bits));
   const uint32_t ib = info.inner_bits, lb = info.leaf_bits;
   const uint32_t inner_size = 1u << (3 * ib), leaf_size = 1u << (3 * lb);
   const uint32_t inner_words = std::max<uint32_t>(inner_size >> 6, 1u);
@@ -208,6 +210,15 @@ inline VoxelGrid<DataT> Deserialize(std::istream& input, HeaderInfo info) {
                grid.deviceContext());
     grid.refresh(true);
   }
+}
+
+/// Build a grid from the stream after its header line (serialization.hpp:159-199).
+template <typename DataT>
+inline VoxelGrid<DataT> Deserialize(std::istream& input, HeaderInfo info) {
+  if (details::demangle(typeid(DataT).name()) != info.type_name)
+    throw std::runtime_error("DataT does not match");
+  VoxelGrid<DataT> grid(info.resolution, uint8_t(info.inner_bits), uint8_t(info.leaf_bits));
+  DeserializeInto(input, info, grid);
   return grid;
 }
 

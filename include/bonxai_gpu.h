@@ -81,6 +81,7 @@ typedef struct bxg_map_stats {
   uint32_t update_count;        /* ProbabilisticMap::_update_count (1..3) */
   uint32_t status;              /* device error bits, 0 if none */
   uint64_t last_ray_visits;     /* counted runs: cells the rays crossed (sum of ray lengths) */
+  uint64_t replays;             /* asynchronous inserts run again after a pool overflow */
 } bxg_map_stats;
 
 /* one scan of a batched insert */

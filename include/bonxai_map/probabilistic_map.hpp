@@ -22,7 +22,12 @@
 
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
+#include <istream>
+#include <ostream>
+#include <stdexcept>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -32,6 +37,7 @@
 #endif
 
 #include "bonxai/bonxai.hpp"
+#include "bonxai/serialization.hpp"
 
 #if defined(__GNUC__) && !defined(__clang__)
 #define BONXAI_MATH_CONSTEXPR constexpr  // GCC folds std::log / std::exp
@@ -176,7 +182,7 @@ class ProbabilisticMap {
                                              BXG_POINTS_F64, o, max_range),
                  _grid.deviceContext());
     }
-    _grid.refresh(true);
+    afterScan(o, max_range);
   }
 
   /// The reference server's ingest pre-step fused into the scan
@@ -195,7 +201,8 @@ class ProbabilisticMap {
     gpu::check(bxg_insert_point_cloud_tf_host(_grid.deviceMap(), _fbuf.data(), sensor_points.size(), 12,
                                               sensor_to_world, max_range),
                _grid.deviceContext());
-    _grid.refresh(true);
+    const double o[3] = {double(sensor_to_world[3]), double(sensor_to_world[7]), double(sensor_to_world[11])};
+    afterScan(o, max_range);
   }
 #ifdef BONXAI_GPU_HAS_EIGEN
   template <typename PointT, typename Allocator>
@@ -220,7 +227,7 @@ class ProbabilisticMap {
     gpu::check(bxg_insert_point_cloud(_grid.deviceMap(), points_dev, n, f64 ? BXG_POINTS_F64 : 0u, o,
                                       max_range),
                _grid.deviceContext());
-    _grid.refresh(true);
+    afterScan(o, max_range);
   }
 
   /// probabilistic_map.cpp:30-41: the endpoint cell is updated now; its ray is
@@ -284,19 +291,84 @@ class ProbabilisticMap {
     gpu::check(bxg_set_update_count(_grid.deviceMap(), v), _grid.deviceContext());
   }
 
+  /// Checkpoint of the map.  The reference has none: its stream
+  /// (serialization.hpp:77-118) holds a grid only, and a map restored from it
+  /// restarts the 1 -> 2 -> 3 scan counter (probabilistic_map.hpp:129), so its
+  /// next scans diverge from the uninterrupted map's.  Written here: the grid
+  /// stream Serialize(grid()) writes, byte for byte (the reference's
+  /// GetHeaderInfo / Deserialize<CellT> read it and stop before what follows),
+  /// then the text line "Bonxai::ProbabilisticMap(update_count=N)\n".  The
+  /// options are not part of it (the caller sets them, as for a new map).
+  /// Throws std::logic_error while addHitPoint / addMissPoint rays wait for the
+  /// next insertPointCloud: they are not part of the checkpoint.
+  void saveCheckpoint(std::ostream& out) const {
+    if (!_pending_pts.empty())
+      throw std::logic_error("addHitPoint/addMissPoint rays are pending: insert a scan before a checkpoint");
+    const uint32_t n = updateCount();
+    Serialize(out, _grid);
+    char line[64];
+    std::snprintf(line, sizeof(line), "Bonxai::ProbabilisticMap(update_count=%u)\n", unsigned(n));
+    out << line;
+    if (!out) throw std::runtime_error("checkpoint write failed");
+  }
+
+  /// Restore a checkpoint written by saveCheckpoint into this map (built with
+  /// the same resolution): its cells and scan counter are replaced, and the
+  /// next insertPointCloud continues as the saved map would have.
+  void loadCheckpoint(std::istream& in) {
+    std::string header;
+    if (!std::getline(in, header)) throw std::runtime_error("empty checkpoint");
+    const HeaderInfo info = GetHeaderInfo(header + "\n");
+    char want[64], got[64];
+    std::snprintf(want, sizeof(want), "%lf", _grid.voxelSize());
+    std::snprintf(got, sizeof(got), "%lf", info.resolution);
+    if (std::string(want) != got) throw std::runtime_error("checkpoint resolution differs from the map's");
+    _grid.clear(CLEAR_MEMORY);
+    _pending_pts.clear();
+    DeserializeInto(in, info, _grid);
+    std::string trailer;
+    unsigned n = 0;
+    char tail = 0;
+    if (!std::getline(in, trailer) ||
+        std::sscanf(trailer.c_str(), "Bonxai::ProbabilisticMap(update_count=%u%c", &n, &tail) != 2 ||
+        tail != ')' || n < 1 || n > 3)
+      throw std::runtime_error("checkpoint trailer missing or malformed");
+    setUpdateCount(n);
+  }
+
  private:
   void addPoint(const Vector3D& p, int hit) {
     const double xyz[3] = {p.x(), p.y(), p.z()};
     _grid.flush();
     gpu::check(bxg_add_point_host(_grid.deviceMap(), xyz, hit), _grid.deviceContext());
-    _grid.refresh(true);
+    // only the endpoint's cell changed; its ray is walked by the next scan
+    _pending_pts.push_back({xyz[0], xyz[1], xyz[2]});
+    _grid.refreshNear(Point3D(xyz[0], xyz[1], xyz[2]), kReachMargin * _grid.voxelSize());
   }
+
+  // After a scan from origin o with range max_range: re-read the cached pages
+  // it may have changed.  Every cell it touched lies on a voxel ray from o's
+  // voxel to an endpoint voxel within max_range of o, or to a pending
+  // addHit/addMissPoint endpoint, so nothing beyond that reach plus a few
+  // voxels (kReachMargin > 2 sqrt(3): both ends' voxel offsets) changed.
+  void afterScan(const double o[3], double max_range) {
+    double reach = max_range;
+    for (const auto& p : _pending_pts) {
+      const double dx = p.x - o[0], dy = p.y - o[1], dz = p.z - o[2];
+      reach = std::max(reach, std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+    _pending_pts.clear();
+    _grid.refreshNear(Point3D(o[0], o[1], o[2]), reach + kReachMargin * _grid.voxelSize());
+  }
+
+  static constexpr double kReachMargin = 4.0;
 
   VoxelGrid<CellT> _grid;
   Options _options;
   mutable VoxelGrid<CellT>::ConstAccessor _accessor;
   std::vector<float> _fbuf;
   std::vector<double> _dbuf;
+  std::vector<Point3D> _pending_pts;  // addHit/addMissPoint endpoints whose rays are due
 };
 
 }  // namespace Bonxai

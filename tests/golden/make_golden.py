@@ -19,6 +19,9 @@ container, where the reference tree and oracle/_ref exist):
                   is feasible)
 
 Usage: python tests/golden/make_golden.py [--full]
+       python tests/golden/make_golden.py --batch-twice
+         (adds to digests.json the 64-map frame inserted twice into each
+          map, the state after two launches of the batched frame)
 """
 import json
 import os
@@ -236,7 +239,31 @@ def make_digests(full):
     return d
 
 
+def make_batch_twice():
+    """Digests of the batched frame's 64 maps after the same scan was inserted
+    twice into each (the second insert meets every cell it touches with
+    update_id 1, so it updates them all again)."""
+    path = os.path.join(HERE, "digests.json")
+    with open(path) as f:
+        d = json.load(f)
+    out = []
+    for i, (pts, origin) in enumerate(scenes.batch_scans()):
+        m = oracle.ProbMap(0.1)
+        m.insert_point_cloud(pts, origin, 30.0)
+        m.insert_point_cloud(pts, origin, 30.0)
+        out.append({"map": i, "digest": list(map(str, m.digest())), "stats": m.stats(),
+                    "update_count": m.update_count})
+        m.close()
+    d["maps"]["batch_twice"] = out
+    with open(path, "w") as f:
+        json.dump(d, f, indent=1, sort_keys=True)
+    print("batch twice done")
+
+
 def main():
+    if "--batch-twice" in sys.argv:
+        make_batch_twice()
+        return
     full = "--full" in sys.argv
     if not os.path.exists(os.path.join(HERE, "apple_xyz.npy")):
         np.save(os.path.join(HERE, "apple_xyz.npy"), read_pcd_xyz(REF_DATA))

@@ -225,7 +225,15 @@ def test_probabilistic_map_dropin(tmp_path):
     script, result = tmp_path / "script.bin", tmp_path / "result.bin"
     script.write_bytes(sc.bytes(struct.pack("<d", res)))
     run("dropin_probmap", script, result)
-    raw = result.read_bytes()
+    check_probmap_result(result.read_bytes(), expect_q, model, res)
+    model.close()
+
+
+def check_probmap_result(raw, expect_q, model, res):
+    """Compare a dropin_probmap result with the oracle model's state: every
+    query op's outputs (expect_q: oracle states and words, in order), then
+    forEachCell, getOccupied/FreeVoxels, the Point3D overload,
+    activeCellsCount and the update count."""
     pos = 0
     for st, word in expect_q:
         n = len(st)
@@ -262,6 +270,75 @@ def test_probabilistic_map_dropin(tmp_path):
     active, upd = struct.unpack_from("<QI", raw, pos)
     assert active == len(want_c)
     assert upd == model.update_count
+
+
+def test_checkpoint_resume_and_local_refresh(tmp_path):
+    """ProbabilisticMap::saveCheckpoint / loadCheckpoint: a map saved after four
+    scans (update_count 2) and restored into a new map continues exactly as
+    the uninterrupted map over three more scans; the checkpoint opens with the
+    bytes of Serialize(grid()).  The scans have a 2 m reach from moving
+    origins, so most cached pages lie outside a scan's reach and are not
+    re-read, and an addHitPoint endpoint 4 m away extends the next scan's
+    reach: the accessors' values after every scan equal the oracle's."""
+    res = 0.05
+    rng = np.random.default_rng(17)
+    model = oracle.ProbMap(res)
+    sc = Script()
+    expect_q = []
+    seq = []  # (points, origin) of every scan, for the sensitivity check
+
+    def insert(pts, origin):
+        origin = tuple(float(np.float32(x)) for x in origin)
+        pts = np.ascontiguousarray(pts, np.float32)
+        sc.op(0, struct.pack("<I4d", len(pts), *origin, 2.0) + pts.tobytes())
+        model.insert_point_cloud(pts, np.array(origin, np.float64), 2.0)
+        seq.append((pts, np.array(origin, np.float64)))
+
+    def query():
+        c, _ = model.cells()
+        q = np.concatenate([c, query_coords(rng, c, 3000)]).astype(np.int32)
+        sc.op(4, sc.coords(q))
+        st = model.classify(q)
+        vals, found = model.grid.lookup(q)
+        expect_q.append((st, np.where(found, vals.view(np.int32).astype(np.int64), -1)))
+
+    room, o1 = scenes.box_scan(n=20_000, seed=11)
+    o2 = o1 + np.array([1.5, 0.5, 0.0])
+    insert(room[:10_000], o1)
+    query()
+    insert(room[10_000:], o2)
+    query()
+    hit = (float(o1[0]) - 4.0, float(o1[1]) + 0.3, float(o1[2]))
+    sc.op(2, struct.pack("<3d", *hit))
+    model.add_hit_point(np.array(hit))
+    insert(room[::3], o1)
+    query()
+    assert model.update_count == 1
+    insert(room[1::3], o2)
+    assert model.update_count == 2
+    sc.op(7)
+    for k, o in enumerate((o1, o2, o1)):
+        insert(room[(k % 3)::2], o)
+        query()
+    script, result = tmp_path / "script.bin", tmp_path / "result.bin"
+    script.write_bytes(sc.bytes(struct.pack("<d", res)))
+    out = run("dropin_probmap", script, result)
+    assert "checkpoint" in out and "update_count 2" in out
+    check_probmap_result(result.read_bytes(), expect_q, model, res)
+    # the scenario is sensitive to the counter: restarting it at 1 after the
+    # fourth scan gives another map
+    alt = oracle.ProbMap(res)
+    for k, (pts, origin) in enumerate(seq[:4]):
+        if k == 2:
+            alt.add_hit_point(np.array(hit))
+        alt.insert_point_cloud(pts, origin, 2.0)
+    alt.update_count = 1
+    for pts, origin in seq[4:]:
+        alt.insert_point_cloud(pts, origin, 2.0)
+    a_c, a_w = alt.cells()
+    m_c, m_w = model.cells()
+    assert not (np.array_equal(a_c, m_c) and np.array_equal(a_w, m_w))
+    alt.close()
     model.close()
 
 
