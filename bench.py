@@ -72,28 +72,32 @@ def reset_maps(ctx, bx, maps):
 
 
 def make_map_sets(ctx, bx, scans, dev, max_range, nsets, budget_bytes):
-    """Up to nsets sets of one map per scan (as many as fit in budget_bytes),
-    each pool reserved from a sizing pass of the frame (1.25x its leaves), so
-    that no timed frame allocates memory or overflows a pool."""
-    sizing = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
-    bx.insert_batch(ctx, [(m, d, o, max_range) for m, d, (_, o) in zip(sizing, dev, scans)])
-    need = []
-    for m in sizing:
-        st = m.stats()
-        need.append((st["leaf_count"], st["root_count"]))
-        m.close()
-    # leaf: 8 mask words, 512 cells of 4 bytes, origin; root: key + 64 entries
-    per_set = sum((int(nl * 1.25) + 8192) * (64 + 2048 + 16) + 2 * (int(nr * 2) + 1024) * 264
-                  for nl, nr in need)
-    nsets = max(1, min(nsets, int(budget_bytes // per_set)))
+    """Up to nsets sets of one map per scan, as many as fit in budget_bytes.
+    Each set is sized by the library's own head-room policy: the frame is
+    integrated into it once eagerly and once through a captured graph (as a
+    caller's first frames would be), and it is emptied after each, so that
+    its pools hold what a fresh frame needs and no timed frame allocates
+    device memory or overflows a pool.  Returns the sets, the leaves of one
+    fresh frame and the device bytes of one set."""
+    descs = lambda maps: [(m, d, o, max_range) for m, d, (_, o) in zip(maps, dev, scans)]
     sets = []
-    for _ in range(nsets):
+    per_set = 0
+    leaves = 0
+    g = bx.ScanGraph(ctx, len(scans), sum(len(p) for p, _ in scans))
+    while len(sets) < nsets and (not sets or (len(sets) + 1) * per_set <= budget_bytes):
         maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
-        for m, (nl, nr) in zip(maps, need):
-            m.reserve(int(nl * 1.25) + 8192, int(nr * 2) + 1024)
+        bx.insert_batch(ctx, descs(maps))
+        leaves = sum(m.stats()["leaf_count"] for m in maps)
+        reset_maps(ctx, bx, maps)
+        g.set_scans(descs(maps))
+        g.launch()
+        check_maps(maps, "sizing frames")
+        reset_maps(ctx, bx, maps)
+        per_set = max(per_set, sum(m.stats()["device_bytes"] for m in maps))
         sets.append(maps)
+    g.close()
     ctx.sync()
-    return sets, sum(nl for nl, _ in need)
+    return sets, leaves, per_set
 
 
 def replays(sets):
@@ -134,6 +138,7 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets):
         g.set_scans(frames[k])
         g.launch()
     ctx.sync()
+    mem0 = ctx.memory()[0]
     fresh_ms = 0.0
     done = rounds = 0
     while done < steps:
@@ -151,6 +156,7 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets):
     for ms in sets:
         check_maps(ms, "timed fresh frames")
     fresh_replays = replays(sets) - replays0
+    fresh_alloc = ctx.memory()[0] - mem0
     # per-frame latency of fresh frames, one frame between two events
     lat_fresh = []
     for k in range(max(steps, 10)):
@@ -204,7 +210,7 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets):
         "points": total_points, "fresh_ms": fresh_ms, "fresh_rounds": rounds, "steady_ms": steady_ms,
         "lat_fresh": lat_fresh, "lat_steady": lat_steady, "fresh_counts": fresh_counts,
         "steady_counts": steady_counts, "kern_fresh": kern_fresh, "kern_steady": kern_steady,
-        "replays_timed": fresh_replays, "replays_all": all_replays,
+        "replays_timed": fresh_replays, "replays_all": all_replays, "alloc_timed": fresh_alloc,
     }
 
 
@@ -293,17 +299,34 @@ def run_lookups(ctx, bx, scenes, reps=20):
     # the memory system's rate for the hits' cell reads alone: as many scattered
     # 4-byte reads from a 64 MiB table (the map's cells take 67 MB)
     hits = int(found.sum())
-    ctx.profile_reset()
-    ctx.profile(True)
-    ctx.gather_probe(GATHER_TABLE, hits, reps=reps).free()
-    ctx.profile(False)
-    gn, gms = ctx.profile_read("k_gather_probe")
-    gather_ms = gms / max(gn, 1)
+    # (the probe library launches on the context's stream but outside its
+    # per-kernel profiler: one warm-up launch, then reps launches back to back
+    # between two events)
+    table = ctx.array(GATHER_TABLE // 4, np.uint32)
+    gout = ctx.array(max(hits, 1), np.uint32)
+
+    def gather():
+        rc = bx.probes().bxg_gather_probe(ctx._h, table.ptr, GATHER_TABLE, hits, gout.ptr)
+        if rc != bx.OK:
+            raise RuntimeError(f"gather probe failed: {rc}")
+
+    gather()
+    ctx.sync()
+    ctx.event(4)
+    for _ in range(reps):
+        gather()
+    ctx.event(5)
+    ctx.sync()
+    gather_ms = ctx.elapsed_ms(4, 5) / reps
+    table.free()
+    gout.free()
     for d in (dc, dv, dq, out_v, out_f):
         d.free()
     g.close()
     nq = len(q)
+    cpu = cpu_grid_baseline(coords, vals, q, cpu_threads())
     return {
+        "cpu_baseline": cpu,
         "lookups": nq, "lookup_ms": avg_ms, "lookups_per_s": nq / (avg_ms / 1e3),
         "hit_frac": float(found.mean()), "inserts": len(coords), "insert_ms": set_ms,
         "gather_ceiling": {"reads": hits, "table_bytes": GATHER_TABLE, "ms": gather_ms,
@@ -392,19 +415,51 @@ def run_sequence(ctx, bx, scenes, nscans=200, cpu_budget_s=3.0):
             "cpu_sample": f"the first {cpu_scans} scans ({cpu_pts} points, {cpu_s:.1f} s)"}
 
 
-def run_serving(ctx, bx, scenes, nreq=800, rate=1500.0, nmaps=16, seed=21):
-    """Serving form: nmaps resident maps at 0.05 m, a Poisson stream of
-    requests -- inserts of 640x480 depth frames (max range 5 m), lookup
-    batches log-uniform in 10k..1M coords, occupied-voxel exports -- admitted
-    by the engine while earlier ones run.  Latency = completion - submission."""
+SERVE_MAPS = 16
+SERVE_FRAMES = 24
+
+
+def serving_trace(nreq, rate, seed=21, nmaps=SERVE_MAPS):
+    """A Poisson request stream: kind (0 insert of a 640x480 depth frame, 1
+    lookup batch of 10k..1M coordinates, log-uniform, 2 occupied-voxel
+    export), target map, lookup size and the gap before each request (all
+    zero for rate None: every request submitted at once).  The lookup
+    coordinates are four fixed sets of 1M, of which a batch takes a prefix."""
     rng = np.random.default_rng(seed)
+    coords = [np.column_stack([rng.integers(-160, 160, 1_000_000), rng.integers(-160, 160, 1_000_000),
+                               rng.integers(-20, 80, 1_000_000)]).astype(np.int32) for _ in range(4)]
+    kinds = rng.choice(3, nreq, p=[0.5, 0.4, 0.1])
+    which = rng.integers(0, nmaps, nreq)
+    sizes = np.exp(rng.uniform(np.log(10_000), np.log(1_000_000), nreq)).astype(np.int64)
+    gaps = rng.exponential(1.0 / rate, nreq) if rate else np.zeros(nreq)
+    return {"coords": coords, "kinds": kinds, "which": which, "sizes": sizes, "gaps": gaps,
+            "rate": rate, "nmaps": nmaps}
+
+
+def latency_summary(infos, span_s):
+    out = {}
+    for kind in ("insert", "lookup", "export"):
+        lat = np.array([i["latency_ms"] for kd, i, _ in infos if kd == kind])
+        # an export's work is the number of cells it wrote, known only once it is done
+        work = sum(i["count"] if kd == "export" else n for kd, i, n in infos if kd == kind)
+        if len(lat):
+            out[kind] = {"count": int(len(lat)), "p50_ms": float(np.percentile(lat, 50)),
+                         "p99_ms": float(np.percentile(lat, 99)), "items_per_s": work / span_s}
+    return out
+
+
+def run_serving(ctx, bx, scenes, trace):
+    """Serving form: 16 resident maps at 0.05 m and the request stream of
+    `trace` -- inserts of 640x480 depth frames (max range 5 m), lookup
+    batches, occupied-voxel exports -- admitted by the engine while earlier
+    ones run.  Latency = completion - submission."""
+    nmaps, nreq = trace["nmaps"], len(trace["kinds"])
     maps = [bx.ProbabilisticMap(ctx, 0.05) for _ in range(nmaps)]
     frames = []
-    for k in range(24):
+    for k in range(SERVE_FRAMES):
         pts, origin = scenes.depth_frame(1000 + k)
         frames.append((ctx.to_device(pts), origin))
-    big = [ctx.to_device(np.column_stack([rng.integers(-160, 160, 1_000_000), rng.integers(-160, 160, 1_000_000),
-                                          rng.integers(-20, 80, 1_000_000)]).astype(np.int32)) for _ in range(4)]
+    big = [ctx.to_device(c) for c in trace["coords"]]
     nout = 2 * 32
     outs = [(ctx.array(1_000_000, np.uint32), ctx.array(1_000_000, np.uint8)) for _ in range(nout)]
     ex = [(ctx.array((8_000_000, 3), np.int32), ctx.array(8_000_000, np.uint32)) for _ in range(4)]
@@ -415,10 +470,7 @@ def run_serving(ctx, bx, scenes, nreq=800, rate=1500.0, nmaps=16, seed=21):
     for w in warm:
         if srv.wait(w)["status"] != 0:
             raise RuntimeError("serving warm-up insert failed")
-    kinds = rng.choice(3, nreq, p=[0.5, 0.4, 0.1])
-    which = rng.integers(0, nmaps, nreq)
-    sizes = np.exp(rng.uniform(np.log(10_000), np.log(1_000_000), nreq)).astype(np.int64)
-    gaps = rng.exponential(1.0 / rate, nreq)
+    kinds, which, sizes, gaps = trace["kinds"], trace["which"], trace["sizes"], trace["gaps"]
     ids = []
     t0 = time.perf_counter()
     t_next = t0
@@ -449,24 +501,88 @@ def run_serving(ctx, bx, scenes, nreq=800, rate=1500.0, nmaps=16, seed=21):
         raise RuntimeError("a serving request failed")
     first = min(i["submit_ns"] for _, i, _ in infos)
     last = max(i["done_ns"] for _, i, _ in infos)
-    out = {"requests": nreq, "offered_rate_per_s": rate, "maps": nmaps, "submit_s": t_sub,
-           "wall_s": t_all, "requests_per_s": nreq / ((last - first) / 1e9),
+    span = (last - first) / 1e9
+    out = {"requests": nreq, "offered_rate_per_s": trace["rate"], "maps": nmaps, "submit_s": t_sub,
+           "wall_s": t_all, "requests_per_s": nreq / span,
            "waves": stats["waves"], "scans_per_wave": stats["wave_scans"] / max(stats["waves"], 1),
            "deferred": stats["deferred"], "retried": stats["retried"]}
-    for kind in ("insert", "lookup", "export"):
-        lat = np.array([i["latency_ms"] for kd, i, _ in infos if kd == kind])
-        # an export's work is the number of cells it wrote, known only once it is done
-        work = sum(i["count"] if kd == "export" else n for kd, i, n in infos if kd == kind)
-        if len(lat):
-            out[kind] = {"count": int(len(lat)), "p50_ms": float(np.percentile(lat, 50)),
-                         "p99_ms": float(np.percentile(lat, 99)),
-                         "items_per_s": work / ((last - first) / 1e9)}
+    out.update(latency_summary(infos, span))
     for d, _ in frames:
         d.free()
     for a in big + [x for pair in outs + ex for x in pair]:
         a.free()
     for m in maps:
         m.close()
+    return out
+
+
+def run_serving_sweep(ctx, bx, scenes, rates, nreq):
+    """The serving form at rising offered rates (None: every request at
+    once), to show where continuous batching engages and where it saturates."""
+    rows = []
+    for rate in rates:
+        r = run_serving(ctx, bx, scenes, serving_trace(nreq, rate))
+        rows.append({k: r[k] for k in ("offered_rate_per_s", "requests", "requests_per_s", "scans_per_wave",
+                                      "waves", "deferred") if k in r}
+                    | {kind: {k: r[kind][k] for k in ("p50_ms", "p99_ms")} for kind in ("insert", "lookup", "export")
+                       if kind in r})
+        log(f"serving at {rate or 'unlimited'} req/s offered: {r['requests_per_s']:.0f} req/s, "
+            f"{r['scans_per_wave']:.2f} scans a wave")
+    return rows
+
+
+def cpu_serving_baseline(scenes, trace, nthreads, budget_s):
+    """The oracle serving the same request stream on the host: the 16 maps
+    split over nthreads worker threads (a map's requests in order on its
+    worker), each map warmed with two frames as on the GPU, requests taken
+    in stream order as fast as the workers go (the CPU cannot keep up with
+    the offered rate, so this is its saturated throughput; latency is each
+    request's own service time).  Stops once a worker has spent budget_s."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    nmaps = trace["nmaps"]
+    frames = [scenes.depth_frame(1000 + k) for k in range(SERVE_FRAMES)]
+    maps = [oracle.ProbMap(0.05) for _ in range(nmaps)]
+    for i, m in enumerate(maps):
+        for j in range(2):
+            pts, o = frames[(2 * i + j) % len(frames)]
+            m.insert_point_cloud(pts, o, 5.0)
+    kinds, which, sizes = trace["kinds"], trace["which"], trace["sizes"]
+
+    def worker(t):
+        done = []
+        t0 = time.perf_counter()
+        for k in range(len(kinds)):
+            if which[k] % nthreads != t:
+                continue
+            if time.perf_counter() - t0 > budget_s:
+                break
+            m = maps[which[k]]
+            s0 = time.perf_counter()
+            if kinds[k] == 0:
+                pts, o = frames[k % len(frames)]
+                m.insert_point_cloud(pts, o, 5.0)
+                kind, n = "insert", len(pts)
+            elif kinds[k] == 1:
+                m.grid.lookup(trace["coords"][k % 4][:int(sizes[k])])
+                kind, n = "lookup", int(sizes[k])
+            else:
+                n = len(m.grid.export(1, 0, sort=False)[0])
+                kind = "export"
+            done.append((kind, {"latency_ms": (time.perf_counter() - s0) * 1e3, "count": n}, n))
+        return done, time.perf_counter() - t0
+
+    with ThreadPoolExecutor(nthreads) as ex:
+        t0 = time.perf_counter()
+        res = list(ex.map(worker, range(nthreads)))
+        wall = time.perf_counter() - t0
+    for m in maps:
+        m.close()
+    infos = [x for d, _ in res for x in d]
+    out = {"requests": len(infos), "requests_per_s": len(infos) / wall, "cores": nthreads, "kind": "port",
+           "sample": f"the first requests of the stream, each worker for at most {budget_s:.0f} s "
+                     f"({len(infos)} requests in {wall:.1f} s), oracle/bx_oracle.c -O2"}
+    out.update(latency_summary(infos, wall))
     return out
 
 
@@ -532,6 +648,57 @@ def cpu_baseline(scans, max_range, budget_s):
     return pts_done / dt, maps_done, pts_done, dt
 
 
+def cpu_grid_baseline(coords, vals, q, nthreads):
+    """Config 2 on the host's cores, over the reference's own VoxelGrid<float>
+    and Accessor where oracle/_ref is built (the loops of the reference's
+    benchmark, benchmark_bonxai.cpp:8-44: setValue per coordinate through one
+    accessor, then value per query through a const accessor), else over the
+    oracle's restatement.  One thread: the 2M inserts, then the 8M lookups.
+    nthreads threads: the lookups split into nthreads slices over the one
+    grid (a const accessor each; reads only), and the inserts split by root
+    key (coordinate >> 5) into one grid per thread, each in input order, so
+    every cell gets its writes in order from one thread."""
+    import oracle
+    from concurrent.futures import ThreadPoolExecutor
+    ref = oracle.ref_available()
+    make = (lambda: oracle.RefGrid(0.05, np.float32)) if ref else (lambda: oracle.Grid(0.05, dtype=np.float32))
+    g = make()
+    t0 = time.perf_counter()
+    g.set_values(coords, vals)
+    t_set = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    g.lookup(q)
+    t_get = time.perf_counter() - t0
+    slices = np.array_split(np.arange(len(q)), nthreads)
+    qs = [np.ascontiguousarray(q[s_]) for s_ in slices]
+    with ThreadPoolExecutor(nthreads) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(g.lookup, qs))
+        t_get_mt = time.perf_counter() - t0
+    g.close()
+    r = coords >> 5
+    owner = ((r[:, 0].astype(np.int64) * 73856093) ^ (r[:, 1].astype(np.int64) * 19349663)
+             ^ (r[:, 2].astype(np.int64) * 83492791)) % nthreads
+    parts = [(np.ascontiguousarray(coords[owner == k]), np.ascontiguousarray(vals[owner == k]))
+             for k in range(nthreads)]
+    grids = [make() for _ in range(nthreads)]
+    with ThreadPoolExecutor(nthreads) as ex:
+        t0 = time.perf_counter()
+        list(ex.map(lambda k: grids[k].set_values(*parts[k]), range(nthreads)))
+        t_set_mt = time.perf_counter() - t0
+    for gg in grids:
+        gg.close()
+    kind = "reference" if ref else "port"
+    return {
+        "inserts_per_s": len(coords) / t_set, "lookups_per_s": len(q) / t_get, "cores": 1, "kind": kind,
+        "threads": {"inserts_per_s": len(coords) / t_set_mt, "lookups_per_s": len(q) / t_get_mt,
+                    "cores": nthreads, "kind": kind},
+        "sample": f"the whole workload: {len(coords)} setValue then {len(q)} value lookups, "
+                  + ("the reference's VoxelGrid<float> compiled from its headers (oracle/_ref)"
+                     if ref else "oracle/bx_oracle.c -O2"),
+    }
+
+
 def cpu_reference_storage(scans, max_range):
     """The same scans with the same integration over the reference's own
     VoxelGrid<Cell> and Accessor (compiled from its headers into oracle/_ref),
@@ -558,7 +725,7 @@ def main():
     ap.add_argument("--maps", type=int, default=64)
     ap.add_argument("--map-sets", type=int, default=20,
                     help="sets of empty maps timed back to back (at most --steps)")
-    ap.add_argument("--map-set-gib", type=int, default=96,
+    ap.add_argument("--map-set-gib", type=int, default=160,
                     help="device memory the map sets may take")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-lookups", action="store_true")
@@ -599,9 +766,10 @@ def main():
     dev = [ctx.to_device(p) for p, _ in scans]
 
     t0 = time.time()
-    sets, frame_leaves = make_map_sets(ctx, bx, scans, dev, max_range, min(args.steps, args.map_sets),
-                                       args.map_set_gib << 30)
-    log(f"{len(sets)} sets of {len(scans)} maps ({frame_leaves} leaves a fresh frame) in {time.time() - t0:.1f}s")
+    sets, frame_leaves, set_bytes = make_map_sets(ctx, bx, scans, dev, max_range,
+                                                  min(args.steps, args.map_sets), args.map_set_gib << 30)
+    log(f"{len(sets)} sets of {len(scans)} maps ({frame_leaves} leaves a fresh frame, "
+        f"{set_bytes / 2**30:.1f} GiB a set) in {time.time() - t0:.1f}s")
     b = run_batched(ctx, bx, scans, dev, args.steps, args.warmup, max_range, sets)
     log(f"batched: fresh {b['fresh_ms'] / args.steps:.3f} ms/frame, steady {b['steady_ms'] / args.steps:.3f} "
         f"ms/frame, replays {b['replays_all']}")
@@ -626,9 +794,15 @@ def main():
         log(f"lidar sequence: {c3['ms_per_scan']:.3f} ms/scan ({c3['rays_per_s'] / 1e6:.0f} M rays/s)")
     sv = None
     if not args.no_serving:
-        sv = run_serving(ctx, bx, scenes, nreq=args.serve_requests, rate=args.serve_rate)
+        sv = run_serving(ctx, bx, scenes, serving_trace(args.serve_requests, args.serve_rate))
         log(f"serving: {sv['requests_per_s']:.0f} req/s, insert p50/p99 "
             f"{sv['insert']['p50_ms']:.2f}/{sv['insert']['p99_ms']:.2f} ms")
+        sv["sweep"] = run_serving_sweep(ctx, bx, scenes, [1500.0, 3000.0, 6000.0, 12000.0, None],
+                                        args.serve_requests // 2)
+        sv["cpu_baseline"] = cpu_serving_baseline(scenes, serving_trace(args.serve_requests, args.serve_rate),
+                                                  cpu_threads(), args.cpu_seconds / 2)
+        log(f"serving on the host: {sv['cpu_baseline']['requests_per_s']:.1f} req/s "
+            f"on {sv['cpu_baseline']['cores']} threads")
 
     npts = b["points"]
     fc, sc_ = b["fresh_counts"], b["steady_counts"]
@@ -694,6 +868,7 @@ def main():
         "visits_per_s": visits / (ms_step / 1e3),
         "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "replays_after_timed_frames": b["replays_timed"],
+        "device_bytes_allocated_by_timed_frames": b["alloc_timed"],
         "kernels_ms": b["kern_fresh"],
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -32,6 +32,7 @@ __all__ = [
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbonxai_gpu.so")
+PROBE_LIB_PATH = os.path.join(_HERE, "libbonxai_probes.so")
 
 OK = 0
 ERR_INVALID_ARG = 1
@@ -164,9 +165,6 @@ ABI = [
     ("bxg_graph_set_scans", _INT, [_P, ctypes.POINTER(ScanDesc), _U32]),
     ("bxg_graph_launch", _INT, [_P]),
     ("bxg_graph_destroy", _INT, [_P]),
-    ("bxg_fp64_probe", _INT, [_P, _P, _P, _P, _U64, _P]),
-    ("bxg_copy_probe", _INT, [_P, _P, _P, _U64]),
-    ("bxg_gather_probe", _INT, [_P, _P, _U64, _U64, _P]),
     ("bxg_server_create", _INT, [_P, ctypes.POINTER(ServerOptions), _PPTR]),
     ("bxg_server_destroy", _INT, [_P]),
     ("bxg_server_drain", _INT, [_P]),
@@ -178,7 +176,31 @@ ABI = [
     ("bxg_server_get_stats", _INT, [_P, ctypes.POINTER(ServerStats)]),
 ]
 
+# measurement probes (include/bonxai_gpu_probes.h), a library of their own
+PROBE_ABI = [
+    ("bxg_fp64_probe", _INT, [_P, _P, _P, _P, _U64, _P]),
+    ("bxg_copy_probe", _INT, [_P, _P, _P, _U64]),
+    ("bxg_gather_probe", _INT, [_P, _P, _U64, _U64, _P]),
+]
+
 _lib = None
+_probes = None
+
+
+def probes():
+    """The loaded probe library (ImportError if it has not been built)."""
+    global _probes
+    if _probes is None:
+        lib()  # the probe library links against the map library
+        if not os.path.exists(PROBE_LIB_PATH):
+            raise ImportError(f"{PROBE_LIB_PATH} is missing: build it with __graft_entry__.build()")
+        handle = ctypes.CDLL(PROBE_LIB_PATH)
+        for name, res, args in PROBE_ABI:
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _probes = handle
+    return _probes
 
 
 def lib():
@@ -302,6 +324,11 @@ class Context:
     def sync(self):
         _check(lib().bxg_ctx_sync(self._h), self, "sync")
 
+    def last_error(self):
+        """The message of the context's last failed call."""
+        msg = lib().bxg_ctx_last_error(self._h)
+        return msg.decode() if msg else ""
+
     def memory(self):
         used, budget = ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().bxg_ctx_memory(self._h, ctypes.byref(used), ctypes.byref(budget)), self)
@@ -357,7 +384,8 @@ class Context:
         dst = self.array(nbytes // 16, np.dtype((np.uint32, 4)))
         try:
             for _ in range(reps):
-                _check(lib().bxg_copy_probe(self._h, src.ptr, dst.ptr, nbytes), self, "copy probe")
+                _check(probes().bxg_copy_probe(self._h, src.ptr, dst.ptr, nbytes), self, "copy probe")
+            self.sync()
         finally:
             src.free()
             dst.free()
@@ -373,8 +401,9 @@ class Context:
         out = self.array(n, np.uint32)
         try:
             for _ in range(reps):
-                _check(lib().bxg_gather_probe(self._h, table.ptr, table_bytes, n, out.ptr), self,
+                _check(probes().bxg_gather_probe(self._h, table.ptr, table_bytes, n, out.ptr), self,
                        "gather probe")
+            self.sync()
         except BaseException:
             out.free()
             raise
@@ -391,7 +420,7 @@ class Context:
         da, db, dc = self.to_device(a), self.to_device(b), self.to_device(c)
         out = self.array((n, 5), np.float64)
         try:
-            _check(lib().bxg_fp64_probe(self._h, da.ptr, db.ptr, dc.ptr, n, out.ptr), self, "fp64 probe")
+            _check(probes().bxg_fp64_probe(self._h, da.ptr, db.ptr, dc.ptr, n, out.ptr), self, "fp64 probe")
             return out.numpy()
         finally:
             for d in (da, db, dc, out):

@@ -2,6 +2,7 @@
 
 Outputs, all in-tree so they travel with the source tree:
   bonxai_amd/libbonxai_gpu.so   HIP library behind include/bonxai_gpu.h (gfx950)
+  bonxai_amd/libbonxai_probes.so  measurement probes behind include/bonxai_gpu_probes.h
   bonxai_amd/libbxscenes.so     seeded synthetic workloads (C)
   bonxai_amd/bin/*              C++ drop-in programs built on include/bonxai*/
 Each step compiles every time it is called.
@@ -54,6 +55,18 @@ def build_gpu_library():
     return out
 
 
+def build_probe_library():
+    """The measurement probes, in a library of their own beside the map
+    library (they run on a map context's stream)."""
+    out = os.path.join(HERE, "libbonxai_probes.so")
+    tmp = out + ".tmp"
+    _run([_hipcc(), *HIP_FLAGS, "-shared", "-I", os.path.join(ROOT, "include"),
+          os.path.join(HERE, "csrc", "bxg_probes.hip"), "-L", HERE, "-lbonxai_gpu",
+          "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_scenes():
     out = os.path.join(HERE, "libbxscenes.so")
     _run(["gcc", "-std=c11", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-Wall",
@@ -82,7 +95,7 @@ def build_dropin_programs():
 
 
 def build_all():
-    outs = [build_gpu_library(), build_scenes()]
+    outs = [build_gpu_library(), build_probe_library(), build_scenes()]
     outs += build_dropin_programs()
     return outs
 

@@ -494,11 +494,15 @@ uint64_t scan_leaf_estimate(const bxg_map* m, uint64_t n, double max_range) {
 }
 
 // keep head-room ahead of a call expected to add up to `leaves` leaves and
-// `roots` roots (estimates; the device reports an overflow if they are low)
+// `roots` roots (estimates; the device reports an overflow if they are low).
+// The leaf estimates are upper bounds already (the sensing ball, twice the
+// largest growth seen, one leaf per cell written), so the pool grows only
+// when it cannot hold the estimate, and then to a quarter beyond it (and at
+// least to twice its size, grow_leaves).
 int ensure_room(bxg_map* m, uint64_t leaves, uint64_t roots) {
   poll_counters(m);
   const uint64_t want_leaves = m->known_leaves + leaves;
-  if (want_leaves > m->leaf_cap * 3 / 4) BXG_TRY(grow_leaves(m, want_leaves * 3 / 2 + 1024));
+  if (want_leaves > m->leaf_cap) BXG_TRY(grow_leaves(m, want_leaves + want_leaves / 4 + 1024));
   const uint64_t want_roots = m->known_roots + roots;
   const uint64_t slack = m->ctx->root_slack;
   if (want_roots * slack > m->root_cap) BXG_TRY(grow_roots(m, want_roots * slack));
@@ -616,55 +620,61 @@ int origin_voxel(bxg_map* m, const double o[3], int32_t out[3]) {
   return BXG_OK;
 }
 
+// Fill the device descriptor of scan `in`, the tag-th of its wave, whose
+// points start at flattened index off.
+int build_desc(bxg_ctx* c, const bxg_scan_desc& in, uint32_t tag, uint64_t off, uint32_t extra_flags,
+               ScanDesc& d) {
+  bxg_map* m = in.map;
+  if (!m || m->ctx != c) return fail(c, BXG_ERR_INVALID_ARG, "scan without a map of this context");
+  if (!is_prob(m)) return fail(c, BXG_ERR_INVALID_ARG, "insertPointCloud needs a probabilistic map");
+  if (in.n && !in.points_dev) return fail(c, BXG_ERR_INVALID_ARG, "null points");
+  if (std::isnan(in.max_range)) return fail(c, BXG_ERR_INVALID_ARG, "max_range is NaN");
+  if (in.n > 0xFFFFFFFFull) return fail(c, BXG_ERR_INVALID_ARG, "scan too large");
+  std::memset(&d, 0, sizeof(d));
+  d.map = m->d;
+  d.points = in.points_dev;
+  d.pt_off = off;
+  d.n = (uint32_t)in.n;
+  const bool f64 = (in.flags & BXG_POINTS_F64) != 0;
+  d.flags = (f64 ? kScanF64 : 0u) | extra_flags;
+  const uint32_t packed = f64 ? 24u : 12u;
+  d.stride = in.stride ? in.stride : packed;
+  if (d.stride < packed || d.stride % (f64 ? 8u : 4u))
+    return fail(c, BXG_ERR_INVALID_ARG, "point stride smaller than a point or misaligned");
+  double origin[3] = {in.origin[0], in.origin[1], in.origin[2]};
+  if (in.sensor_to_world) {
+    if (f64) return fail(c, BXG_ERR_INVALID_ARG, "the ingest transform takes float points");
+    d.flags |= kScanTransform;
+    std::memcpy(d.tf, in.sensor_to_world, sizeof(d.tf));
+    // the sensor position: the matrix's translation (bonxai_server.cpp:172-175)
+    for (int k = 0; k < 3; ++k) origin[k] = (double)in.sensor_to_world[4 * k + 3];
+  }
+  d.fx = origin[0];
+  d.fy = origin[1];
+  d.fz = origin[2];
+  d.max_range = in.max_range;
+  d.r2 = in.max_range * in.max_range;  // probabilistic_map.hpp:145
+  int32_t o[3];
+  BXG_TRY(origin_voxel(m, origin, o));
+  d.ox = o[0];
+  d.oy = o[1];
+  d.oz = o[2];
+  d.tag = tag;
+  if (!(extra_flags & kScanNoRays) && !m->extra.empty()) {
+    d.extra = m->d_extra;
+    d.nextra = (uint32_t)m->extra.size();
+  }
+  return BXG_OK;
+}
+
 // Fill the device descriptors of one wave of scans (distinct maps).
 int build_descs(bxg_ctx* c, const bxg_scan_desc* scans, uint32_t nscans, uint32_t extra_flags,
                 std::vector<ScanDesc>& out, uint64_t* total) {
   out.resize(nscans);
   uint64_t off = 0;
   for (uint32_t s = 0; s < nscans; ++s) {
-    const bxg_scan_desc& in = scans[s];
-    bxg_map* m = in.map;
-    if (!m || m->ctx != c) return fail(c, BXG_ERR_INVALID_ARG, "scan without a map of this context");
-    if (!is_prob(m)) return fail(c, BXG_ERR_INVALID_ARG, "insertPointCloud needs a probabilistic map");
-    if (in.n && !in.points_dev) return fail(c, BXG_ERR_INVALID_ARG, "null points");
-    if (std::isnan(in.max_range)) return fail(c, BXG_ERR_INVALID_ARG, "max_range is NaN");
-    ScanDesc& d = out[s];
-    std::memset(&d, 0, sizeof(d));
-    d.map = m->d;
-    d.points = in.points_dev;
-    d.pt_off = off;
-    d.n = (uint32_t)in.n;
-    const bool f64 = (in.flags & BXG_POINTS_F64) != 0;
-    d.flags = (f64 ? kScanF64 : 0u) | extra_flags;
-    const uint32_t packed = f64 ? 24u : 12u;
-    d.stride = in.stride ? in.stride : packed;
-    if (d.stride < packed || d.stride % (f64 ? 8u : 4u))
-      return fail(c, BXG_ERR_INVALID_ARG, "point stride smaller than a point or misaligned");
-    double origin[3] = {in.origin[0], in.origin[1], in.origin[2]};
-    if (in.sensor_to_world) {
-      if (f64) return fail(c, BXG_ERR_INVALID_ARG, "the ingest transform takes float points");
-      d.flags |= kScanTransform;
-      std::memcpy(d.tf, in.sensor_to_world, sizeof(d.tf));
-      // the sensor position: the matrix's translation (bonxai_server.cpp:172-175)
-      for (int k = 0; k < 3; ++k) origin[k] = (double)in.sensor_to_world[4 * k + 3];
-    }
-    d.fx = origin[0];
-    d.fy = origin[1];
-    d.fz = origin[2];
-    d.max_range = in.max_range;
-    d.r2 = in.max_range * in.max_range;  // probabilistic_map.hpp:145
-    int32_t o[3];
-    BXG_TRY(origin_voxel(m, origin, o));
-    d.ox = o[0];
-    d.oy = o[1];
-    d.oz = o[2];
-    d.tag = s;
-    if (!(extra_flags & kScanNoRays) && !m->extra.empty()) {
-      d.extra = m->d_extra;
-      d.nextra = (uint32_t)m->extra.size();
-    }
-    if (in.n > 0xFFFFFFFFull) return fail(c, BXG_ERR_INVALID_ARG, "scan too large");
-    off += in.n;
+    BXG_TRY(build_desc(c, scans[s], s, off, extra_flags, out[s]));
+    off += scans[s].n;
   }
   *total = off;
   if (off >= (1ull << 31)) return fail(c, BXG_ERR_INVALID_ARG, "too many points in one call");
@@ -945,6 +955,7 @@ int known_leaf_count(bxg_map* m, uint32_t* out) {
 }
 
 }  // namespace
+
 
 // ====================================================================
 //                                C ABI
@@ -2110,39 +2121,6 @@ int bxg_graph_destroy(bxg_graph* g) {
   dfree(g->d_hdr);
   delete g;
   return BXG_OK;
-}
-
-// ------------------------------------------------------------ checks
-
-int bxg_copy_probe(bxg_ctx* c, const void* src, void* dst, uint64_t bytes) {
-  if (!c || (bytes && (!src || !dst)) || (bytes % 16)) return BXG_ERR_INVALID_ARG;
-  std::lock_guard<std::recursive_mutex> lk(c->mu);
-  if (!bytes) return BXG_OK;
-  const uint64_t n = bytes / 16;
-  BXG_TRY(launch(c, "k_copy_probe", k_copy_probe, (uint32_t)std::min<uint64_t>(nblocks(n), 8192), kBlock,
-                 (const uint4*)src, (uint4*)dst, n));
-  return ctx_sync(c);
-}
-
-int bxg_gather_probe(bxg_ctx* c, const void* table, uint64_t table_bytes, uint64_t n, void* out) {
-  const uint64_t words = table_bytes / 4;
-  if (!c || table_bytes % 4 || words == 0 || (words & (words - 1)) || (n && (!table || !out)))
-    return BXG_ERR_INVALID_ARG;
-  std::lock_guard<std::recursive_mutex> lk(c->mu);
-  if (!n) return BXG_OK;
-  if (n > (uint64_t)UINT32_MAX * kBlock) return BXG_ERR_INVALID_ARG;
-  BXG_TRY(launch(c, "k_gather_probe", k_gather_probe, lookup_blocks(n), kBlock, (const uint32_t*)table,
-                 words - 1, n, (uint32_t*)out));
-  return ctx_sync(c);
-}
-
-int bxg_fp64_probe(bxg_ctx* c, const double* a, const double* b, const double* cc, uint64_t n,
-                   double* out) {
-  if (!c || (n && (!a || !b || !cc || !out))) return BXG_ERR_INVALID_ARG;
-  std::lock_guard<std::recursive_mutex> lk(c->mu);
-  if (!n) return BXG_OK;
-  BXG_TRY(launch(c, "k_fp64_probe", k_fp64_probe, nblocks(n), kBlock, a, b, cc, n, out));
-  return ctx_sync(c);
 }
 
 }  // extern "C"

@@ -1882,42 +1882,6 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t n, uint64_t v) {
     p[i] = v;
 }
 
-// a plain 16-byte-per-lane copy: calibration of the HBM byte counters
-__global__ void __launch_bounds__(256) k_copy_probe(const uint4* __restrict__ src,
-                                                    uint4* __restrict__ dst, uint64_t n) {
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * blockDim.x)
-    dst[i] = src[i];
-}
-
-// n scattered 4-byte reads from a table of mask + 1 words (a power of two),
-// kLookups a thread as in k_lookup: the rate the memory system gives the cell
-// reads of a lookup batch whose coordinates come in no particular order
-__device__ __forceinline__ uint64_t probe_mix(uint64_t x) {
-  x ^= x >> 31;
-  x *= 0x7fb5d329728ea185ull;
-  x ^= x >> 27;
-  x *= 0x81dadef4bc2dd44dull;
-  return x ^ (x >> 33);
-}
-
-__global__ void __launch_bounds__(256) k_gather_probe(const uint32_t* __restrict__ table,
-                                                      uint64_t mask, uint64_t n,
-                                                      uint32_t* __restrict__ out) {
-  const uint64_t base = (uint64_t)blockIdx.x * blockDim.x * kLookups + threadIdx.x;
-  uint32_t v[kLookups];
-#pragma unroll
-  for (uint32_t j = 0; j < kLookups; ++j) {
-    const uint64_t i = base + (uint64_t)j * blockDim.x;
-    v[j] = i < n ? table[probe_mix(i) & mask] : 0u;
-  }
-#pragma unroll
-  for (uint32_t j = 0; j < kLookups; ++j) {
-    const uint64_t i = base + (uint64_t)j * blockDim.x;
-    if (i < n) out[i] = v[j];
-  }
-}
-
 // ----------------------------------------------------- leaf page I/O
 
 // Copy whole leaves (mask words + cell bytes) named by their origin coords
@@ -1971,21 +1935,6 @@ __global__ void __launch_bounds__(256) k_leaf_origins(const MapDev* __restrict__
   out[3 * k] = o.x;
   out[3 * k + 1] = o.y;
   out[3 * k + 2] = o.z;
-}
-
-// ------------------------------------------------ arithmetic self-check
-
-// The FP64 operations of the scan front end, element-wise, for the
-// correctly-rounded checks of sqrt, division, multiply-add order and floor.
-__global__ void k_fp64_probe(const double* __restrict__ a, const double* __restrict__ b,
-                             const double* __restrict__ c, uint64_t n, double* __restrict__ out) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  out[5 * i + 0] = __builtin_sqrt(a[i]);
-  out[5 * i + 1] = a[i] / b[i];
-  out[5 * i + 2] = c[i] + ((a[i] / b[i]) * c[i]);
-  out[5 * i + 3] = __builtin_floor(a[i] * b[i]);
-  out[5 * i + 4] = (a[i] * a[i] + b[i] * b[i]) + c[i] * c[i];
 }
 
 }  // namespace bxg

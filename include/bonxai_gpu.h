@@ -329,24 +329,6 @@ int bxg_server_wait(bxg_server* s, uint64_t id, bxg_request_info* info);
 int bxg_server_query(bxg_server* s, uint64_t id, bxg_request_info* info);
 int bxg_server_get_stats(bxg_server* s, bxg_server_stats* out);
 
-/* ---------------------------------------------------------- checks */
-
-/* a plain device-to-device copy kernel (16 bytes a lane), bytes % 16 == 0:
- * the known-size reference that scales the HBM byte counters of a profile */
-int bxg_copy_probe(bxg_ctx* ctx, const void* src_dev, void* dst_dev, uint64_t bytes);
-
-/* n scattered 4-byte reads from a device table of table_bytes (a power of two,
- * at least 4), written in order to out_dev (n words): the rate the memory
- * system gives to randomly placed cell reads, the part of a lookup batch in
- * caller order that no kernel can coalesce */
-int bxg_gather_probe(bxg_ctx* ctx, const void* table_dev, uint64_t table_bytes, uint64_t n,
-                     void* out_dev);
-
-/* element-wise FP64 operations of the scan front end on device arrays:
- * out[5i..5i+4] = sqrt(a), a/b, c + (a/b)*c, floor(a*b), (a*a + b*b) + c*c */
-int bxg_fp64_probe(bxg_ctx* ctx, const double* a_dev, const double* b_dev, const double* c_dev,
-                   uint64_t n, double* out_dev);
-
 #ifdef __cplusplus
 }
 #endif

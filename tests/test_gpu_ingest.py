@@ -57,6 +57,10 @@ def test_ingest_host_matches_oracle(ctx):
         om.insert_point_cloud_tf(cloud, tf, 5.0)
         assert_same_cells(m.cells(), om.cells(), f"frame {k}")
     print(f"points whose endpoint depends on the transform's association: {amb}", flush=True)
+    # these frames hold points whose endpoint voxel differs between the two
+    # float associations (1, 3 and 5 of them), so the equal maps show the GPU
+    # computes the SSE association, not the scalar one
+    assert amb > 0
     assert m.update_count == om.update_count
     m.close()
     om.close()
@@ -150,3 +154,57 @@ def test_ingest_graph_frames_match_oracle(ctx):
         m.close()
         om.close()
         d.free()
+
+
+def lidar_sensor_cloud(rng, pts, origin):
+    """A LiDAR scan moved into its own sensor frame, as PointXYZ, with a few
+    NaN entries."""
+    out = np.zeros((len(pts), 4), np.float32)
+    out[:, :3] = pts - origin.astype(np.float32)
+    bad = rng.choice(len(out), len(out) // 100, replace=False)
+    out[bad, 1] = np.nan
+    return out
+
+
+def test_async_tf_inserts_replay_after_overflow(ctx):
+    """Asynchronous transformed inserts of device points whose head-room
+    estimate is far too small (two tiny scans first teach it a small growth;
+    each LiDAR scan, an eighth of its beams at 20 m, then needs about 18k
+    leaves, twice the pool they leave): the scans that
+    overflowed the node pool run again when the map is confirmed, with the
+    matrices they were given.  Each matrix array is overwritten with NaN
+    right after its call returns, so a replay that read the caller's memory
+    instead of the library's copy would move every point off the map."""
+    import bonxai_amd as bx
+    import oracle
+    from bonxai_amd import scenes
+    rng = np.random.default_rng(14)
+    scans = list(scenes.lidar_sequence(4))
+    m = bx.ProbabilisticMap(ctx, 0.1)
+    om = oracle.ProbMap(0.1)
+    tiny = lidar_sensor_cloud(rng, *scans[0])[:300]
+    tf0 = rigid(rng)
+    for _ in range(2):
+        m.insert_point_cloud_tf(tiny, tf0, 20.0)
+        om.insert_point_cloud_tf(tiny, tf0, 20.0)
+    cap0 = m.stats()["leaf_capacity"]
+    dev = []
+    for pts, origin in scans:
+        cloud = lidar_sensor_cloud(rng, pts[::8], origin)
+        tf = rigid(rng)
+        d = ctx.to_device(cloud)
+        dev.append(d)
+        handed = tf.copy()
+        m.insert_point_cloud_tf(d, handed, 20.0, sync=False)
+        handed[:] = np.nan
+        om.insert_point_cloud_tf(cloud, tf, 20.0)
+    assert m.sync() == 0
+    st = m.stats()
+    print(f"leaf capacity {cap0} -> {st['leaf_capacity']}, replays {st['replays']}", flush=True)
+    assert st["leaf_count"] > cap0 and st["replays"] > 0
+    assert_same_cells(m.cells(), om.cells(), "async transformed inserts after overflow")
+    assert m.update_count == om.update_count
+    for d in dev:
+        d.free()
+    m.close()
+    om.close()
