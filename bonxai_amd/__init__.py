@@ -159,6 +159,7 @@ ABI = [
     ("bxg_add_point_host", _INT, [_P, _PD, _INT]),
     ("bxg_insert_point_cloud_tf", _INT, [_P, _P, _U64, _U32, _P, ctypes.c_double]),
     ("bxg_insert_point_cloud_tf_host", _INT, [_P, _P, _U64, _U32, _P, ctypes.c_double]),
+    ("bxg_insert_point_cloud_noalloc", _INT, [_P, _P, _U64, _U32, _PD, _D]),
     ("bxg_classify", _INT, [_P, _P, _U64, _P]),
     ("bxg_classify_host", _INT, [_P, _P, _U64, _P]),
     ("bxg_graph_create", _INT, [_P, _U32, _U64, _PPTR]),
@@ -640,6 +641,17 @@ class ProbabilisticMap(VoxelGrid):
             return
         _check(lib().bxg_insert_point_cloud_host(self._h, _ptr(p), len(p), flags, o, float(max_range)),
                self.ctx, "insert_point_cloud")
+
+    def insert_point_cloud_noalloc(self, points, origin, max_range):
+        """insertPointCloud through the allocation-free entry point: device
+        points (at most 524288), the library's static scratch, the pools as
+        reserved; blocks.  Returns the status code (ERR_OUT_OF_POOL when the
+        pools cannot hold the scan: no cell changed)."""
+        if not isinstance(points, DeviceArray):
+            raise TypeError("the allocation-free insert takes device points")
+        p, flags = _as_points(points)
+        o = (ctypes.c_double * 3)(*[float(x) for x in origin])
+        return lib().bxg_insert_point_cloud_noalloc(self._h, p.ptr, len(p), flags, o, float(max_range))
 
     def insert_point_cloud_tf(self, points, sensor_to_world, max_range, sync=True):
         """The reference server's ingest pre-step fused into the scan

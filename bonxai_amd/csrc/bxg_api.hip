@@ -956,6 +956,135 @@ int known_leaf_count(bxg_map* m, uint32_t* out) {
 
 }  // namespace
 
+// ------------------------------------------------ allocation-free scans
+//
+// The scratch of bxg_insert_point_cloud_noalloc: the scan pipeline's per-call
+// buffers for at most BXG_NOALLOC_MAX_POINTS points (endpoint records and
+// table, winner lists, bucket counts and cursors, batch header and
+// descriptor), in a file-scope device array.  Every GPU the library runs on
+// holds its own copy from the library's load to the end of the process.  Host
+// code never uses the array's name (a host-side stand-in, not a device
+// address): the device address comes from hipGetSymbolAddress, taken once per
+// device with that device current, at the first bxg_ctx_create on it.
+namespace {
+
+constexpr uint64_t kNaPoints = BXG_NOALLOC_MAX_POINTS;
+constexpr uint64_t kNaEpSlots = kNaPoints * 2;  // endpoint table at load factor <= 1/2
+constexpr uint64_t kNaOffHdr = 0;
+constexpr uint64_t kNaOffDesc = 256;
+constexpr uint64_t kNaOffCounters = 1024;
+constexpr uint64_t kNaOffHist = 4096;
+constexpr uint64_t kNaOffOffs = kNaOffHist + (uint64_t)kMaxBuckets * 4;
+constexpr uint64_t kNaOffRec = kNaOffOffs + (uint64_t)kMaxBuckets * 4;
+constexpr uint64_t kNaOffTpos = kNaOffRec + kNaPoints * sizeof(int4);
+constexpr uint64_t kNaOffWinners = kNaOffTpos + kNaPoints * 4;
+constexpr uint64_t kNaOffWkey = kNaOffWinners + kNaPoints * 4;
+constexpr uint64_t kNaOffEp = kNaOffWkey + kNaPoints * 4;
+constexpr uint64_t kNaBytes = kNaOffEp + kNaEpSlots * sizeof(EpSlot);
+static_assert(sizeof(BatchHdr) <= kNaOffDesc - kNaOffHdr, "batch header slot");
+static_assert(sizeof(ScanDesc) <= kNaOffCounters - kNaOffDesc, "descriptor slot");
+static_assert(kCounterWords * 4 <= kNaOffHist - kNaOffCounters, "counter slot");
+static_assert(kNaOffRec % 256 == 0 && kNaOffEp % 256 == 0, "aligned buffers");
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;                      // guards g_na_ptr
+uint8_t* g_na_ptr[kNaMaxDevices] = {};   // device address of the array, per device
+std::mutex g_na_run[kNaMaxDevices];      // one call at a time per device
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNaBytes];
+
+namespace {
+
+// Take the array's device address on `device` (current) once.
+int na_scratch_init(int device) {
+  if (device < 0 || device >= kNaMaxDevices) return BXG_ERR_NOT_SUPPORTED;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (g_na_ptr[device]) return BXG_OK;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return BXG_ERR_HIP;
+  g_na_ptr[device] = (uint8_t*)ptr;
+  return BXG_OK;
+}
+
+uint8_t* na_scratch(const bxg_ctx* c) {
+  if (c->device < 0 || c->device >= kNaMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  return g_na_ptr[c->device];
+}
+
+// the scan pipeline's buffers inside the array, for a call of n points
+ScanScratch na_layout(uint8_t* base, uint64_t n, uint64_t* ep_slots) {
+  ScanScratch sc{};
+  sc.rec = (int4*)(base + kNaOffRec);
+  sc.tpos = (uint32_t*)(base + kNaOffTpos);
+  sc.winners = (uint32_t*)(base + kNaOffWinners);
+  sc.wkey = (uint32_t*)(base + kNaOffWkey);
+  sc.hist = (uint32_t*)(base + kNaOffHist);
+  sc.offs = (uint32_t*)(base + kNaOffOffs);
+  sc.counters = (uint32_t*)(base + kNaOffCounters);
+  sc.ep = (EpSlot*)(base + kNaOffEp);
+  const uint64_t slots = std::max<uint64_t>(pow2_at_least(n), 1 << 12) * 2;  // <= kNaEpSlots
+  sc.ep_mask = (uint32_t)(slots - 1);
+  *ep_slots = slots;
+  return sc;
+}
+
+// One scan of n device points into m on the array's buffers, waited for.
+// The map's pools are used as they are: a scan they cannot hold writes no
+// cell and returns BXG_ERR_OUT_OF_POOL.
+int insert_noalloc(bxg_map* m, const bxg_scan_desc& in) {
+  bxg_ctx* c = m->ctx;
+  uint8_t* base = na_scratch(c);
+  if (!base) return fail(c, BXG_ERR_HIP, "the allocation-free scratch has no device address");
+  std::lock_guard<std::mutex> run(g_na_run[c->device]);
+  const uint32_t overflow = kStOutOfPool | kStRootFull;
+  // earlier asynchronous inserts of the map: confirmed here when none of them
+  // overflowed (repairing an overflow grows the pools, which this call never
+  // does)
+  if (!m->pending.empty()) {
+    BXG_TRY(read_counters(m));
+    if (m->h_ctr->status & overflow)
+      return fail(c, BXG_ERR_MAP_FAILED,
+                  "an earlier asynchronous insert of the map ran out of nodes: bxg_map_sync repairs it "
+                  "(and grows the pools)");
+    BXG_TRY(settle(m));
+  }
+  poll_counters(m);
+  if (m->ctr_pending) BXG_TRY(read_counters(m));
+  if (m->h_ctr->status)
+    return fail(c, BXG_ERR_MAP_FAILED, "the map holds a device error; call bxg_map_clear_error");
+  if (!m->extra.empty())
+    return fail(c, BXG_ERR_NOT_SUPPORTED,
+                "addHitPoint/addMissPoint rays are due: bxg_insert_point_cloud traces them");
+  ScanDesc d;
+  BXG_TRY(build_desc(c, in, 0, 0, 0u, d));
+  const BatchHdr hdr{1u, tag_bits_for(1), in.n};
+  uint64_t slots = 0;
+  const ScanScratch sc = na_layout(base, in.n, &slots);
+  BatchHdr* d_hdr = (BatchHdr*)(base + kNaOffHdr);
+  ScanDesc* d_desc = (ScanDesc*)(base + kNaOffDesc);
+  // the array keeps the last call's contents: clear what starts empty
+  BXG_TRY(fill64(c, (uint64_t*)sc.ep, slots * 2, ~0ull));
+  BXG_TRY(fill32(c, sc.hist, kMaxBuckets, 0));
+  BXG_TRY(fill32(c, sc.counters, kCounterWords, 0));
+  BXG_TRY(launch(c, "k_na_setup", k_na_setup, 1, 64, hdr, d, d_hdr, d_desc, m->ctr,
+                 c->counting ? 1u : 0u));
+  BXG_TRY(enqueue_scan_kernels(c, d_hdr, d_desc, sc, in.n, 1, false));
+  BXG_TRY(read_counters(m));  // waits for the scan
+  const uint32_t st = m->h_ctr->status;
+  if (st & ~overflow) return map_status(m);
+  if (st) {
+    BXG_TRY(launch(c, "k_na_recover", k_na_recover, 1, 64, m->ctr, m->leaf_cap));
+    BXG_TRY(read_counters(m));
+    return fail(c, BXG_ERR_OUT_OF_POOL,
+                "the map's pools cannot hold the scan (no cell changed): reserve more and call again");
+  }
+  return BXG_OK;
+}
+
+}  // namespace
 
 // ====================================================================
 //                                C ABI
@@ -1009,6 +1138,9 @@ int bxg_ctx_create(int device, uint64_t budget_bytes, bxg_ctx** out) {
     if (hipEventCreate(&c->ev[i]) != hipSuccess) rc = BXG_ERR_HIP;
   if (rc == BXG_OK) rc = fill32(c, c->d_counters, kCounterWords, 0);
   if (rc == BXG_OK) rc = ctx_sync(c);
+  // the allocation-free scratch's address on this device (an error here
+  // leaves the other entry points working; the allocation-free one reports it)
+  if (rc == BXG_OK) na_scratch_init(device);
   if (rc != BXG_OK) {
     bxg_ctx_destroy(c);
     return rc;
@@ -1876,6 +2008,17 @@ int bxg_insert_point_cloud_tf(bxg_map* m, const float* points, uint64_t n, uint3
   if (!m || !sensor_to_world) return BXG_ERR_INVALID_ARG;
   std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
   return insert_async(m, scan_of(m, points, n, 0, stride, nullptr, sensor_to_world, max_range));
+}
+
+int bxg_insert_point_cloud_noalloc(bxg_map* m, const void* points, uint64_t n, uint32_t flags,
+                                   const double origin[3], double max_range) {
+  if (!m || !origin) return BXG_ERR_INVALID_ARG;
+  std::lock_guard<std::recursive_mutex> lk(m->ctx->mu);
+  if (n > kNaPoints)
+    return fail(m->ctx, BXG_ERR_INVALID_ARG,
+                "more points than the allocation-free scratch holds (BXG_NOALLOC_MAX_POINTS = " +
+                    std::to_string(kNaPoints) + ")");
+  return insert_noalloc(m, scan_of(m, points, n, flags, 0, origin, nullptr, max_range));
 }
 
 // Run one wave, then re-run (alone, after growing its pools) every scan

@@ -1315,6 +1315,31 @@ __global__ void __launch_bounds__(256) k_scan_new_masks(const BatchHdr* __restri
   }
 }
 
+// Allocation-free scans (bxg_insert_point_cloud_noalloc): the batch header
+// and the one scan descriptor, passed by value, are written into the
+// caller-side scratch; a counted call also clears the map's scan statistics.
+__global__ void k_na_setup(BatchHdr hdr, ScanDesc desc, BatchHdr* __restrict__ hdr_out,
+                           ScanDesc* __restrict__ desc_out, MapCounters* __restrict__ ctr,
+                           uint32_t reset_stats) {
+  if (threadIdx.x != 0) return;
+  *hdr_out = hdr;
+  *desc_out = desc;
+  if (reset_stats) {
+    ctr->stat_rays = 0;
+    ctr->stat_ray_updates = 0;
+    ctr->stat_ray_visits = 0;
+  }
+}
+
+// After a scan that ran out of nodes (it wrote no cell): the overflow bits
+// are cleared and the bump counter, which ran past the pool, comes back to
+// the pool's size.  The leaves allocated below it stay, every cell off.
+__global__ void k_na_recover(MapCounters* __restrict__ ctr, uint32_t leaf_cap) {
+  if (threadIdx.x != 0) return;
+  ctr->status &= ~(kStOutOfPool | kStRootFull);
+  ctr->leaf_count = min(ctr->leaf_count, leaf_cap);
+}
+
 // Phase 0 (per scan): the leaf count at the start of the scan, so that later
 // phases can tell the leaves the scan creates from older ones.
 __global__ void k_scan_begin(const BatchHdr* __restrict__ hdr, const ScanDesc* __restrict__ sd) {

@@ -247,6 +247,25 @@ int bxg_insert_point_cloud_tf(bxg_map* map, const float* points_dev, uint64_t n,
 /* same, host points; blocks and grows the pools as bxg_insert_point_cloud_host */
 int bxg_insert_point_cloud_tf_host(bxg_map* map, const float* points, uint64_t n, uint32_t stride,
                                    const float sensor_to_world[16], double max_range);
+/* Allocation-free form of bxg_insert_point_cloud.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code that
+ * must not call hipMalloc, such as a tight serving loop or a harness that
+ * forbids it).  Same scan and results, but the call's scratch is a static
+ * device buffer of the library (32 MiB on each GPU it runs on, held
+ * until the process ends) for at most BXG_NOALLOC_MAX_POINTS points, and the
+ * map's pools are used as reserved (bxg_map_reserve): nothing is allocated
+ * or grown.  Blocks until the scan is done.  Returns BXG_ERR_INVALID_ARG for
+ * more points than the maximum; BXG_ERR_OUT_OF_POOL if the pools cannot hold
+ * the scan (no cell changed and update_count did not advance; the error is
+ * cleared, so reserve more and call again); BXG_ERR_NOT_SUPPORTED while
+ * addHitPoint/addMissPoint rays are due (bxg_insert_point_cloud traces
+ * them); BXG_ERR_MAP_FAILED if an earlier asynchronous insert of the map
+ * overflowed (bxg_map_sync repairs that, growing the pools).  Every call on
+ * one GPU uses that GPU's one buffer: the library runs such calls one at a
+ * time per GPU, and other users of the buffer would race with them. */
+#define BXG_NOALLOC_MAX_POINTS 524288u
+int bxg_insert_point_cloud_noalloc(bxg_map* map, const void* points_dev, uint64_t n, uint32_t flags,
+                                   const double origin[3], double max_range);
 /* many scans in one pass over a flattened point queue; scans naming the
  * same map run in input order.  Blocks; a scan whose map ran out of nodes is
  * re-run after the pool grows (its failed run left the cells untouched). */
