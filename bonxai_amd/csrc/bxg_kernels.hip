@@ -133,6 +133,22 @@ __device__ __forceinline__ uint32_t wave_bump(uint32_t* counter, bool pred) {
   return mine;
 }
 
+// The lowest lane of this lane's run: the maximal block of consecutive lanes
+// that all have `on` set and equal (k1, k2).  Lanes of a wavefront that share
+// a table entry and sit side by side let that lane do the entry's atomics
+// for all of them.  Every lane must call it; the result means something only
+// where `on` is set.
+__device__ __forceinline__ int run_leader(bool on, uint64_t k1, uint32_t k2) {
+  const uint32_t lane = lane_id();
+  const uint64_t p1 = __shfl_up(k1, 1u);
+  const uint32_t p2 = __shfl_up(k2, 1u);
+  const uint64_t onmask = __ballot(on);
+  const bool prev_on = lane > 0 && ((onmask >> (lane - 1)) & 1ull);
+  const bool start = on && !(prev_on && p1 == k1 && p2 == k2);
+  const uint64_t upto = __ballot(start) & (~0ull >> (63u - lane));
+  return upto ? 63 - __clzll(upto) : (int)lane;
+}
+
 // ------------------------------------------------------------ map view
 
 // A register copy of a map header with global-address-space pointers.
@@ -191,31 +207,33 @@ __device__ __forceinline__ uint32_t root_hash(uint64_t key, uint32_t mask) {
 }
 
 // Find-or-insert of a root key (lock-free CAS insert, linear probing).
-// Returns the slot, or -1 (and sets kStRootFull) after max_probe probes.
-// The roots a wavefront inserts are counted with one atomic per map
-// (wave_bump), so every active lane must call it.
-__device__ int32_t root_insert(const MapView& m, uint64_t key) {
+// Returns the slot, or -1 (and sets kStRootFull) after max_probe probes;
+// `inserted` tells whether this lane's CAS created it (the caller counts it).
+__device__ int32_t root_probe(const MapView& m, uint64_t key, bool& inserted) {
   uint32_t h = root_hash(key, m.root_mask);
-  int32_t slot = -1;
-  bool inserted = false;
+  inserted = false;
   for (uint32_t p = 0; p < m.max_probe; ++p) {
     uint64_t k = m.root_keys[h];
     if (k == kEmptyKey) {
       k = cas64((uint64_t*)&m.root_keys[h], kEmptyKey, key);
       if (k == kEmptyKey) {
         inserted = true;
-        slot = (int32_t)h;
-        break;
+        return (int32_t)h;
       }
     }
-    if (k == key) {
-      slot = (int32_t)h;
-      break;
-    }
+    if (k == key) return (int32_t)h;
     h = (h + 1) & m.root_mask;
   }
+  or32(&m.ctr->status, kStRootFull);
+  return -1;
+}
+
+// root_probe with the roots a wavefront inserts counted by one atomic per
+// map (wave_bump), so every active lane must call it.
+__device__ int32_t root_insert(const MapView& m, uint64_t key) {
+  bool inserted;
+  const int32_t slot = root_probe(m, key, inserted);
   wave_bump(&m.ctr->root_count, inserted);
-  if (slot < 0) or32(&m.ctr->status, kStRootFull);
   return slot;
 }
 
@@ -669,84 +687,118 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
                                                        const ScanDesc* __restrict__ sd,
                                                        ScanScratch sc) {
   const uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gi >= hdr->total) return;
-  const uint32_t s = find_scan(sd, hdr->nscans, gi);
+  const uint64_t total = hdr->total;
+  if (gi - lane_id() >= total) return;  // the whole wavefront lies past the last point
+  // Every lane of a live wavefront stays to the end: the root, leaf and
+  // endpoint claims below are wavefront collectives.
+  const bool live = gi < total;
+  const uint32_t s = live ? find_scan(sd, hdr->nscans, gi) : 0u;
   const ScanDesc& d = sd[s];
   const MapView m = load_view(d.map);
-  const uint64_t i = gi - d.pt_off;
-  double tx, ty, tz;
-  const uint8_t BXG_G* pb = gp((const uint8_t*)d.points) + i * d.stride;
-  if (d.flags & kScanF64) {
-    const double BXG_G* p = (const double BXG_G*)pb;
-    tx = p[0];
-    ty = p[1];
-    tz = p[2];
-  } else {
-    const float BXG_G* p = (const float BXG_G*)pb;
-    float x = p[0], y = p[1], z = p[2];
-    if (d.flags & kScanTransform) {
-      // bonxai_server.cpp:147-153 drops points with a non-finite coordinate
-      if (!(__builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z))) {
-        sc.rec[gi] = make_int4(0, 0, 0, (int32_t)(s << 16));
-        return;
-      }
-      // pcl::transformPointCloud in float (bonxai_server.cpp:166-170), no contraction
-      const float* t = d.tf;
-      const float nx = t[0] * x + (t[1] * y + (t[2] * z + t[3]));
-      const float ny = t[4] * x + (t[5] * y + (t[6] * z + t[7]));
-      const float nz = t[8] * x + (t[9] * y + (t[10] * z + t[11]));
-      x = nx;
-      y = ny;
-      z = nz;
-    }
-    tx = (double)x;
-    ty = (double)y;
-    tz = (double)z;
-  }
-  const double vx = tx - d.fx, vy = ty - d.fy, vz = tz - d.fz;
-  const double sq = (vx * vx + vy * vy) + vz * vz;
-  double qx = tx, qy = ty, qz = tz;
+  int4 rec = make_int4(0, 0, 0, (int32_t)(s << 16));
+  bool valid = false;
+  int32_t ex = 0, ey = 0, ez = 0;
   uint32_t flags = kRecValid;
-  if (d.flags & (kScanForceHit | kScanForceMiss)) {
-    if (d.flags & kScanForceHit) flags |= kRecHit;
-  } else if (sq >= d.r2) {
-    const double sr = __builtin_sqrt(sq);
-    qx = d.fx + ((vx / sr) * d.max_range);
-    qy = d.fy + ((vy / sr) * d.max_range);
-    qz = d.fz + ((vz / sr) * d.max_range);
-  } else {
-    flags |= kRecHit;
+  if (live) {
+    const uint64_t i = gi - d.pt_off;
+    double tx, ty, tz;
+    bool finite = true;
+    const uint8_t BXG_G* pb = gp((const uint8_t*)d.points) + i * d.stride;
+    if (d.flags & kScanF64) {
+      const double BXG_G* p = (const double BXG_G*)pb;
+      tx = p[0];
+      ty = p[1];
+      tz = p[2];
+    } else {
+      const float BXG_G* p = (const float BXG_G*)pb;
+      float x = p[0], y = p[1], z = p[2];
+      if (d.flags & kScanTransform) {
+        // bonxai_server.cpp:147-153 drops points with a non-finite coordinate
+        finite = __builtin_isfinite(x) && __builtin_isfinite(y) && __builtin_isfinite(z);
+        // pcl::transformPointCloud in float (bonxai_server.cpp:166-170), no contraction
+        const float* t = d.tf;
+        const float nx = t[0] * x + (t[1] * y + (t[2] * z + t[3]));
+        const float ny = t[4] * x + (t[5] * y + (t[6] * z + t[7]));
+        const float nz = t[8] * x + (t[9] * y + (t[10] * z + t[11]));
+        x = nx;
+        y = ny;
+        z = nz;
+      }
+      tx = (double)x;
+      ty = (double)y;
+      tz = (double)z;
+    }
+    if (finite) {
+      const double vx = tx - d.fx, vy = ty - d.fy, vz = tz - d.fz;
+      const double sq = (vx * vx + vy * vy) + vz * vz;
+      double qx = tx, qy = ty, qz = tz;
+      if (d.flags & (kScanForceHit | kScanForceMiss)) {
+        if (d.flags & kScanForceHit) flags |= kRecHit;
+      } else if (sq >= d.r2) {
+        const double sr = __builtin_sqrt(sq);
+        qx = d.fx + ((vx / sr) * d.max_range);
+        qy = d.fy + ((vy / sr) * d.max_range);
+        qz = d.fz + ((vz / sr) * d.max_range);
+      } else {
+        flags |= kRecHit;
+      }
+      const double ax = __builtin_floor(qx * m.inv_res);
+      const double ay = __builtin_floor(qy * m.inv_res);
+      const double az = __builtin_floor(qz * m.inv_res);
+      const double lim = (double)(1 << (20 + m.log2n));
+      if (ax >= -lim && ax < lim && ay >= -lim && ay < lim && az >= -lim && az < lim) {
+        valid = true;
+        ex = (int32_t)ax;
+        ey = (int32_t)ay;
+        ez = (int32_t)az;
+      } else {
+        or32(&m.ctr->status, kStCoordRange);
+      }
+    }
   }
-  const double ax = __builtin_floor(qx * m.inv_res);
-  const double ay = __builtin_floor(qy * m.inv_res);
-  const double az = __builtin_floor(qz * m.inv_res);
-  const double lim = (double)(1 << (20 + m.log2n));
-  if (!(ax >= -lim && ax < lim && ay >= -lim && ay < lim && az >= -lim && az < lim)) {
-    or32(&m.ctr->status, kStCoordRange);
-    sc.rec[gi] = make_int4(0, 0, 0, (int32_t)(s << 16));
-    return;
+  // The endpoint's root, then its leaf (created if missing), then its
+  // first-index claim.  Neighbouring points of a scan usually share a root,
+  // a leaf and often a voxel, and neighbouring points sit in neighbouring
+  // lanes: each run of lanes with the same key does the table work once, on
+  // its lowest lane (run_leader), and the others take the result from it.
+  uint64_t rkey = 0;
+  if (valid) pack_root(ex, ey, ez, m.log2n, &rkey);
+  const int rl = run_leader(valid, rkey, s);
+  int32_t slot = -1;
+  bool rnew = false;
+  if (valid && rl == (int)lane_id()) slot = root_probe(m, rkey, rnew);
+  wave_bump(&m.ctr->root_count, rnew);
+  slot = __shfl(slot, rl);
+  if (valid && slot < 0) {
+    rec = make_int4(ex, ey, ez, (int32_t)(s << 16));
+    valid = false;
   }
-  const int32_t ex = (int32_t)ax, ey = (int32_t)ay, ez = (int32_t)az;
-  uint64_t rkey;
-  pack_root(ex, ey, ez, m.log2n, &rkey);
-  const int32_t slot = root_insert(m, rkey);
-  if (slot < 0) {
-    sc.rec[gi] = make_int4(ex, ey, ez, (int32_t)(s << 16));
-    return;
+  uint32_t ci = 0, li = 0;
+  if (valid) {
+    ci = inner_index(ex, ey, ez, m.lb, m.ib);
+    li = leaf_index(ex, ey, ez, m.lb);
   }
-  const uint32_t ci = inner_index(ex, ey, ez, m.lb, m.ib);
-  const uint32_t li = leaf_index(ex, ey, ez, m.lb);
+  const int ll = run_leader(valid, ((uint64_t)(uint32_t)slot << 32) | ci, s);
+  uint32_t* cp = valid ? (uint32_t*)&m.children[(uint64_t)slot * m.isz + ci] : nullptr;
+  uint32_t seen = kPendingChild;
+  bool won = false;
+  if (valid && ll == (int)lane_id()) won = leaf_claim(cp, seen);
   const int32_t lmask = ~((1 << m.lb) - 1);
-  const uint32_t leaf = leaf_request(m, (uint32_t)slot, ci, ex & lmask, ey & lmask, ez & lmask);
-  bool marked = true;  // a leaf created in this scan has all cells zero: id 0
-  if (leaf < kPendingChild) {
+  leaf_publish(m, won, cp, (uint32_t)slot, ex & lmask, ey & lmask, ez & lmask);
+  // a leaf this wavefront created has every cell zero (update_id 0): PENDING
+  const uint32_t leaf = __shfl(won ? kPendingChild : seen, ll);
+  bool marked = valid;
+  if (valid && leaf < kPendingChild) {
     const uint32_t w = ((const uint32_t BXG_G*)m.cells)[(uint64_t)leaf * m.lsz + li];
     marked = cell_id(w) != map_cur(m);
   }
-  if (marked) {
-    const uint64_t key = ep_key(d.tag, (uint32_t)slot, ci, li);
+  // first point of each voxel: lanes are in point order, so a run's lowest
+  // lane holds its smallest index, the one the endpoint table keeps
+  const uint64_t key = marked ? ep_key(d.tag, (uint32_t)slot, ci, li) : 0ull;
+  const int el = run_leader(marked, key, 0u);
+  uint32_t tp = kNone;
+  if (marked && el == (int)lane_id()) {
     uint32_t h = (uint32_t)mix64(key) & sc.ep_mask;
-    uint32_t tp = kNone;
     for (uint32_t p = 0; p <= sc.ep_mask; ++p) {
       uint64_t k = gp(&sc.ep[h])->key;
       if (k == kEmptyKey) {
@@ -760,6 +812,9 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
       }
       h = (h + 1) & sc.ep_mask;
     }
+  }
+  tp = __shfl(tp, el);
+  if (marked) {
     if (tp == kNone) {
       or32(&m.ctr->status, kStScratchFull);
     } else {
@@ -767,7 +822,8 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
       sc.tpos[gi] = tp;
     }
   }
-  sc.rec[gi] = make_int4(ex, ey, ez, (int32_t)(flags | (s << 16)));
+  if (valid) rec = make_int4(ex, ey, ez, (int32_t)(flags | (s << 16)));
+  if (live) sc.rec[gi] = rec;
 }
 
 // Lanes of a wavefront that share a key form one group: one atomic add of
