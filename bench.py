@@ -110,7 +110,7 @@ def check_maps(maps, what):
             raise RuntimeError(f"a map failed during the {what}")
 
 
-def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets):
+def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets, steady=True):
     """The batched form, one captured graph launch per frame of 64 scans.
     Fresh frames (the workload as stated, and `value`): each timed step
     integrates the frame into a set of 64 empty maps; up to len(sets) steps run
@@ -127,8 +127,10 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets):
     ctx.counting(True)
     bx.insert_batch(ctx, descs[0])
     fresh_counts = frame_stats(sets[0])
-    bx.insert_batch(ctx, descs[0])
-    steady_counts = frame_stats(sets[0])
+    steady_counts = None
+    if steady:
+        bx.insert_batch(ctx, descs[0])
+        steady_counts = frame_stats(sets[0])
     ctx.counting(False)
     g = bx.ScanGraph(ctx, len(scans), total_points)
     # fresh frames, warm-up then timed
@@ -167,8 +169,29 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets):
         g.launch()
         ctx.event(3)
         lat_fresh.append(ctx.elapsed_ms(2, 3))
-    # steady frames into set 0, which holds the frame after the loop above
     check_maps(sets[0], "latency frames")
+    # kernel durations, eager launches with per-kernel events
+    names = ("k_scan_begin", "k_scan_classify", "k_scan_winners", "k_scan_bucket_reduce",
+             "k_scan_bucket_scan", "k_scan_scatter", "k_scan_ray_leaves", "k_scan_endpoints",
+             "k_scan_rays", "k_scan_new_endpoints", "k_scan_new_masks", "k_scan_finalize")
+    nprof = 3
+    ctx.profile_reset()
+    for _ in range(nprof):
+        reset_maps(ctx, bx, sets[0])
+        ctx.profile(True)
+        bx.insert_batch(ctx, descs[0])
+        ctx.profile(False)
+    kern_fresh = {k: ctx.profile_read(k)[1] / max(ctx.profile_read(k)[0], 1) for k in names}
+    out = {
+        "points": total_points, "fresh_ms": fresh_ms, "fresh_rounds": rounds, "lat_fresh": lat_fresh,
+        "fresh_counts": fresh_counts, "kern_fresh": kern_fresh, "replays_timed": fresh_replays,
+        "alloc_timed": fresh_alloc, "steady_ms": None,
+    }
+    if not steady:
+        g.close()
+        out["replays_all"] = replays(sets) - replays0
+        return out
+    # steady frames into set 0, which holds the frame after the eager frames above
     g.set_scans(frames[0])
     for _ in range(warmup):
         g.launch()
@@ -186,19 +209,6 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets):
         ctx.event(3)
         lat_steady.append(ctx.elapsed_ms(2, 3))
     check_maps(sets[0], "steady latency frames")
-    all_replays = replays(sets) - replays0
-    # kernel durations, eager launches with per-kernel events
-    names = ("k_scan_begin", "k_scan_classify", "k_scan_winners", "k_scan_bucket_reduce",
-             "k_scan_bucket_scan", "k_scan_scatter", "k_scan_ray_leaves", "k_scan_endpoints",
-             "k_scan_rays", "k_scan_new_masks", "k_scan_finalize")
-    nprof = 3
-    ctx.profile_reset()
-    for _ in range(nprof):
-        reset_maps(ctx, bx, sets[0])
-        ctx.profile(True)
-        bx.insert_batch(ctx, descs[0])
-        ctx.profile(False)
-    kern_fresh = {k: ctx.profile_read(k)[1] / max(ctx.profile_read(k)[0], 1) for k in names}
     ctx.profile_reset()
     ctx.profile(True)
     for _ in range(nprof):
@@ -206,12 +216,9 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets):
     ctx.profile(False)
     kern_steady = {k: ctx.profile_read(k)[1] / max(ctx.profile_read(k)[0], 1) for k in names}
     g.close()
-    return {
-        "points": total_points, "fresh_ms": fresh_ms, "fresh_rounds": rounds, "steady_ms": steady_ms,
-        "lat_fresh": lat_fresh, "lat_steady": lat_steady, "fresh_counts": fresh_counts,
-        "steady_counts": steady_counts, "kern_fresh": kern_fresh, "kern_steady": kern_steady,
-        "replays_timed": fresh_replays, "replays_all": all_replays, "alloc_timed": fresh_alloc,
-    }
+    out.update({"steady_ms": steady_ms, "lat_steady": lat_steady, "steady_counts": steady_counts,
+                "kern_steady": kern_steady, "replays_all": replays(sets) - replays0})
+    return out
 
 
 def run_plain(ctx, bx, scans, dev, steps, warmup, max_range, sets):
@@ -730,6 +737,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-lookups", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-steady", action="store_true",
+                    help="skip the steady frames (so that a kernel trace holds fresh frames only)")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the single box scan and the 200-scan LiDAR sequence")
@@ -770,9 +779,10 @@ def main():
                                                   min(args.steps, args.map_sets), args.map_set_gib << 30)
     log(f"{len(sets)} sets of {len(scans)} maps ({frame_leaves} leaves a fresh frame, "
         f"{set_bytes / 2**30:.1f} GiB a set) in {time.time() - t0:.1f}s")
-    b = run_batched(ctx, bx, scans, dev, args.steps, args.warmup, max_range, sets)
-    log(f"batched: fresh {b['fresh_ms'] / args.steps:.3f} ms/frame, steady {b['steady_ms'] / args.steps:.3f} "
-        f"ms/frame, replays {b['replays_all']}")
+    b = run_batched(ctx, bx, scans, dev, args.steps, args.warmup, max_range, sets, steady=not args.no_steady)
+    log(f"batched: fresh {b['fresh_ms'] / args.steps:.3f} ms/frame, steady "
+        + (f"{b['steady_ms'] / args.steps:.3f} ms/frame" if b["steady_ms"] is not None else "not run")
+        + f", replays {b['replays_all']}")
     p = None
     if not args.no_plain:
         p = run_plain(ctx, bx, scans, dev, args.steps, args.warmup, max_range, sets)
@@ -805,22 +815,20 @@ def main():
             f"on {sv['cpu_baseline']['cores']} threads")
 
     npts = b["points"]
-    fc, sc_ = b["fresh_counts"], b["steady_counts"]
+    fc, sc_ = b["fresh_counts"], b.get("steady_counts")
     ms_step = b["fresh_ms"] / args.steps
     rays_s = npts / (ms_step / 1e3)
-    ms_steady = b["steady_ms"] / args.steps
+    steady = b["steady_ms"] is not None
+    ms_steady = b["steady_ms"] / args.steps if steady else None
     # voxel updates per frame: endpoint updates (one per traced ray) + ray
     # updates (every distinct cell a ray crossed whose update_id was not cur)
     upd = fc["rays"] + fc["ray_updates"]
-    upd_steady = sc_["rays"] + sc_["ray_updates"]
     # algorithmic bytes of one k_scan_rays launch: per ray its 16-byte record
     # and 4-byte list entry; per updated cell one 4-byte read and one 4-byte
     # write (every distinct cell once)
     k_ms = b["kern_fresh"]["k_scan_rays"]
     alg_bytes = 20 * fc["rays"] + 8 * fc["ray_updates"]
     achieved = alg_bytes / (k_ms / 1e3) / 1e9
-    k_ms_st = b["kern_steady"]["k_scan_rays"]
-    alg_bytes_st = 20 * sc_["rays"] + 8 * sc_["ray_updates"]
     # the whole frame on the integration model: 12 B a point read, 8 B per
     # updated cell, 64 B of mask words per new leaf
     frame_bytes = 12 * npts + 8 * upd + 64 * fc["leaves"]
@@ -834,7 +842,6 @@ def main():
     # endpoint visit per point
     visits = fc["ray_visits"] + npts
     lat = np.array(b["lat_fresh"])
-    lat_st = np.array(b["lat_steady"])
     out = {
         "metric": METRIC,
         "value": rays_s,
@@ -881,20 +888,6 @@ def main():
             "achieved": frame_bytes / (ms_step / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": frame_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
         },
-        "steady": {
-            "what": "the same frame re-inserted back to back into the maps that hold it",
-            "rays_per_s": npts / (ms_steady / 1e3), "ms_per_frame": ms_steady,
-            "voxel_updates_per_s": upd_steady / (ms_steady / 1e3),
-            "ray_updates_per_frame": sc_["ray_updates"],
-            "frame_latency_ms": {"p50": float(np.percentile(lat_st, 50)),
-                                 "p99": float(np.percentile(lat_st, 99))},
-            "kernels_ms": b["kern_steady"],
-            "roofline": {"bound": "hbm", "kernel": "k_scan_rays", "alg_bytes_per_launch": alg_bytes_st,
-                         "achieved": alg_bytes_st / (k_ms_st / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
-                         "frac": alg_bytes_st / (k_ms_st / 1e3) / 1e9 / HBM_PEAK_GBS,
-                         "traffic": measured_traffic("k_scan_rays_steady")},
-            "replays": b["replays_all"] - b["replays_timed"],
-        },
         "cpu_baseline": {
             "value": cpu_rate, "unit": "rays/s", "cores": 1, "kind": "port",
             "sample": f"the first {cpu_maps} scans of the frame ({cpu_pts} points, {cpu_dt:.1f} s), each "
@@ -912,6 +905,25 @@ def main():
                       f"one empty map per thread, {nthr} threads",
         },
     }
+    if steady:
+        upd_steady = sc_["rays"] + sc_["ray_updates"]
+        k_ms_st = b["kern_steady"]["k_scan_rays"]
+        alg_bytes_st = 20 * sc_["rays"] + 8 * sc_["ray_updates"]
+        lat_st = np.array(b["lat_steady"])
+        out["steady"] = {
+            "what": "the same frame re-inserted back to back into the maps that hold it",
+            "rays_per_s": npts / (ms_steady / 1e3), "ms_per_frame": ms_steady,
+            "voxel_updates_per_s": upd_steady / (ms_steady / 1e3),
+            "ray_updates_per_frame": sc_["ray_updates"],
+            "frame_latency_ms": {"p50": float(np.percentile(lat_st, 50)),
+                                 "p99": float(np.percentile(lat_st, 99))},
+            "kernels_ms": b["kern_steady"],
+            "roofline": {"bound": "hbm", "kernel": "k_scan_rays", "alg_bytes_per_launch": alg_bytes_st,
+                         "achieved": alg_bytes_st / (k_ms_st / 1e3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "frac": alg_bytes_st / (k_ms_st / 1e3) / 1e9 / HBM_PEAK_GBS,
+                         "traffic": measured_traffic("k_scan_rays_steady")},
+            "replays": b["replays_all"] - b["replays_timed"],
+        }
     if p is not None:
         plain_rays_s = npts / (p["fresh_ms"] / args.steps / 1e3)
         plain_steady = npts / (p["steady_ms"] / args.steps / 1e3)
@@ -920,7 +932,8 @@ def main():
                         "scan_latency_ms": {"p50": float(np.percentile(p["lat"], 50)),
                                             "p99": float(np.percentile(p["lat"], 99))}}
         out["batched_over_plain"] = rays_s / plain_rays_s
-        out["steady"]["batched_over_plain"] = (npts / (ms_steady / 1e3)) / plain_steady
+        if steady:
+            out["steady"]["batched_over_plain"] = (npts / (ms_steady / 1e3)) / plain_steady
     if lk is not None:
         # algorithmic bytes of a lookup, the declared model: 12-byte coordinate,
         # 8-byte mask word and 4-byte cell read, 4-byte value written (root and

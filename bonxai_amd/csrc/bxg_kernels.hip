@@ -575,8 +575,12 @@ __device__ __forceinline__ void leaves_step(const MapView& m, LeafWalk& lw, Root
   bool won[kLeafRun];
 #pragma unroll
   for (int j = 0; j < kLeafRun; ++j) {
+    // neighbouring rays in neighbouring lanes often reach the same missing
+    // leaf at the same step: the lowest lane of each such run claims it
+    const bool missing = j < n && v[j] == kEmptyChild;
+    const int lead = run_leader(missing, (uint64_t)cp[j], 0u);
     uint32_t seen;
-    won[j] = (j < n && v[j] == kEmptyChild) ? leaf_claim(cp[j], seen) : false;
+    won[j] = (missing && lead == (int)lane_id()) ? leaf_claim(cp[j], seen) : false;
   }
 #pragma unroll
   for (int j = 0; j < kLeafRun; ++j) leaf_publish(m, won[j], cp[j], (uint32_t)slot, lx[j], ly[j], lz[j]);
@@ -756,9 +760,9 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
       }
     }
   }
-  // The endpoint's root, then its leaf (created if missing), then its
-  // first-index claim.  Neighbouring points of a scan usually share a root,
-  // a leaf and often a voxel, and neighbouring points sit in neighbouring
+  // The endpoint's root (created if missing), then its leaf (looked up),
+  // then its first-index claim.  Neighbouring points of a scan usually share
+  // a root and often a voxel, and neighbouring points sit in neighbouring
   // lanes: each run of lanes with the same key does the table work once, on
   // its lowest lane (run_leader), and the others take the result from it.
   uint64_t rkey = 0;
@@ -778,15 +782,10 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
     ci = inner_index(ex, ey, ez, m.lb, m.ib);
     li = leaf_index(ex, ey, ez, m.lb);
   }
-  const int ll = run_leader(valid, ((uint64_t)(uint32_t)slot << 32) | ci, s);
-  uint32_t* cp = valid ? (uint32_t*)&m.children[(uint64_t)slot * m.isz + ci] : nullptr;
-  uint32_t seen = kPendingChild;
-  bool won = false;
-  if (valid && ll == (int)lane_id()) won = leaf_claim(cp, seen);
-  const int32_t lmask = ~((1 << m.lb) - 1);
-  leaf_publish(m, won, cp, (uint32_t)slot, ex & lmask, ey & lmask, ez & lmask);
-  // a leaf this wavefront created has every cell zero (update_id 0): PENDING
-  const uint32_t leaf = __shfl(won ? kPendingChild : seen, ll);
+  // the endpoint's leaf as it was before the scan (the leaf pass creates the
+  // winners' endpoint leaves; nothing creates leaves in this kernel): a
+  // missing leaf means a cell with update_id 0, never cur
+  const uint32_t leaf = valid ? *gp(&m.children[(uint64_t)slot * m.isz + ci]) : kEmptyChild;
   bool marked = valid;
   if (valid && leaf < kPendingChild) {
     const uint32_t w = ((const uint32_t BXG_G*)m.cells)[(uint64_t)leaf * m.lsz + li];
@@ -979,26 +978,39 @@ __global__ void __launch_bounds__(256) k_scan_scatter(const BatchHdr* __restrict
 __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const ScanScratch& sc,
                                           uint32_t unit, uint32_t lg, MapView& m, RayWalk& w,
                                           RootCache& rc, int32_t& cur, uint32_t& lbase,
-                                          bool check_status, bool& first) {
+                                          bool leaf_pass, bool& first) {
   const uint32_t idx = unit >> lg, seg = unit & ((1u << lg) - 1u);
   first = seg == 0;
   const uint32_t gi = sc.winners[idx];
   const int4 r = sc.rec[gi];
   const ScanDesc& d = sd[(uint32_t)r.w >> 16];
-  if (d.flags & kScanNoRays) return false;
-  if (r.x == d.ox && r.y == d.oy && r.z == d.oz) return false;
+  const bool no_rays = (d.flags & kScanNoRays) != 0;
+  const bool at_origin = r.x == d.ox && r.y == d.oy && r.z == d.oz;
+  // the update pass walks cells [0, mx) of the ray (none for an endpoint in
+  // the origin's voxel or a scan without rays); the leaf pass also the
+  // endpoint's cell mx, so that it creates the winner's endpoint leaf with
+  // the ray's (only that cell for a scan without rays)
+  if (!leaf_pass && (no_rays || at_origin)) return false;
+  if (leaf_pass && no_rays && seg != 0) return false;
   m = load_view(d.map);
-  if (check_status && (map_status(m) & kFatal)) return false;
+  if (!leaf_pass && (map_status(m) & kFatal)) return false;
   cur = map_cur(m);
   lbase = *gp(&m.ctr->leaf_base);
   w.init(d.ox, d.oy, d.oz, r.x, r.y, r.z);
-  if (lg) {
+  const int32_t last = w.mx + (leaf_pass ? 1 : 0);
+  int32_t k0 = 0, k1 = last;
+  if (no_rays) {
+    k0 = w.mx;
+  } else if (lg) {
     const int64_t n = 1ll << lg;
-    const int32_t k0 = (int32_t)(((int64_t)w.mx * seg + n - 1) >> lg);
-    const int32_t k1 = (int32_t)(((int64_t)w.mx * (seg + 1) + n - 1) >> lg);
-    if (k0 >= k1) return false;
-    w.seek(k0, k1);
+    k0 = (int32_t)(((int64_t)last * seg + n - 1) >> lg);
+    k1 = (int32_t)(((int64_t)last * (seg + 1) + n - 1) >> lg);
   }
+  if (k0 >= k1) return false;
+  if (k0 == 0)
+    w.kend = k1;  // from the origin (and mx may be 0: no division)
+  else
+    w.seek(k0, k1);
   rc.reset();
   return true;
 }
@@ -1097,7 +1109,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       const uint32_t take = want < avail ? want : avail;
       const uint32_t rank = (uint32_t)__popcll(need & lanes_below());
       if (!have && rank < take) {
-        have = start_ray(sd, sc, chunk_pos + rank, lg, m, w, rc, cur, lbase, kMode != 0, first);
+        have = start_ray(sd, sc, chunk_pos + rank, lg, m, w, rc, cur, lbase, kMode == 0, first);
         nupd = 0;
         if (kMode == 0 && have) lw.init(w, m.lb);
         if (kMode != 0 && have) next_leaf = walk_leaf(m, w, rc);

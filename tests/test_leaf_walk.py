@@ -29,7 +29,7 @@ def leaf_walk(o, e, lb, k0, k1, cap=None):
     m = max(d)
     L = 1 << lb
     lm = L - 1
-    c0 = _cell(o, d, s, m, k0)
+    c0 = _cell(o, d, s, m, k0) if m else tuple(o)  # m = 0: the endpoint is the origin's voxel
     st = []
     for a in range(3):
         n = (2 * k0 * d[a] + m) // (2 * m) if k0 else 0
@@ -84,6 +84,31 @@ def test_short_rays_equal_cell_walk():
             if not want or want[-1][0] != leaf:
                 want.append((leaf, k))
         assert leaf_walk(o, e, lb, k0, k1) == want, (o, e, lb, k0, k1)
+
+
+def test_leaf_pass_reaches_the_endpoint_leaf():
+    """The leaf pass walks cells [k0, m + 1) of a winner's ray, its m cells and
+    then the endpoint (cell m is e), so that the endpoint's leaf is created
+    with the ray's; also for an endpoint in the origin's voxel (m = 0) and for
+    the endpoint-only walk [m, m + 1) of an addHitPoint / addMissPoint scan."""
+    rng = random.Random(7)
+    for _ in range(3000):
+        lb = rng.choice([1, 2, 3, 4])
+        o = [rng.randint(-300, 300) for _ in range(3)]
+        e = [o[i] + rng.randint(-40, 40) for i in range(3)] if rng.random() < 0.9 else list(o)
+        d = [abs(e[i] - o[i]) for i in range(3)]
+        s = [(-1 if e[i] < o[i] else 1) for i in range(3)]
+        m = max(d)
+        cells = [_cell(o, d, s, m, k) for k in range(m + 1)] if m else [tuple(o)]
+        assert cells[-1] == tuple(e)
+        r = rng.random()
+        k0 = 0 if r < 0.4 else (m if r < 0.6 else rng.randint(0, m))
+        want = []
+        for k in range(k0, m + 1):
+            leaf = tuple(v & ~((1 << lb) - 1) for v in cells[k])
+            if not want or want[-1][0] != leaf:
+                want.append((leaf, k))
+        assert leaf_walk(o, e, lb, k0, m + 1) == want, (o, e, lb, k0)
 
 
 def test_long_rays_against_closed_form():
