@@ -78,11 +78,9 @@ struct bxg_ctx {
 
   // bulk-op scratch
   uint64_t op_cap = 0;
-  uint64_t* op_keys = nullptr;
-  uint32_t* op_first = nullptr;
-  uint32_t* op_last = nullptr;
-  uint32_t op_mask = 0;
+  OpSlot* op_slots = nullptr;  // op_cap * 2 slots
   uint32_t* op_tpos = nullptr;
+  uint32_t* op_leaf = nullptr;
   uint8_t* op_prebit = nullptr;
 
   // device staging of host-memory calls
@@ -566,40 +564,38 @@ int ensure_scan_scratch(bxg_ctx* c, uint64_t points, uint32_t nscans) {
 int ensure_op_scratch(bxg_ctx* c, uint64_t n) {
   if (n <= c->op_cap) return BXG_OK;
   BXG_TRY(ctx_sync(c));
-  dfree(c->op_keys);
-  dfree(c->op_first);
-  dfree(c->op_last);
+  dfree(c->op_slots);
   dfree(c->op_tpos);
+  dfree(c->op_leaf);
   dfree(c->op_prebit);
+  c->op_slots = nullptr;
+  c->op_tpos = c->op_leaf = nullptr;
+  c->op_prebit = nullptr;
+  c->op_cap = 0;
   const uint64_t cap = std::max<uint64_t>(pow2_at_least(n), 1 << 12);
   if (cap > (1ull << 31)) return fail(c, BXG_ERR_INVALID_ARG, "too many cells in one call");
-  const uint64_t tcap = cap * 2;
   void* p;
-  BXG_TRY(dmalloc(c, &p, tcap * 8));
-  c->op_keys = (uint64_t*)p;
-  BXG_TRY(dmalloc(c, &p, tcap * 4));
-  c->op_first = (uint32_t*)p;
-  BXG_TRY(dmalloc(c, &p, tcap * 4));
-  c->op_last = (uint32_t*)p;
+  BXG_TRY(dmalloc(c, &p, cap * 2 * sizeof(OpSlot)));
+  c->op_slots = (OpSlot*)p;
   BXG_TRY(dmalloc(c, &p, cap * 4));
   c->op_tpos = (uint32_t*)p;
+  BXG_TRY(dmalloc(c, &p, cap * 4));
+  c->op_leaf = (uint32_t*)p;
   BXG_TRY(dmalloc(c, &p, cap));
   c->op_prebit = (uint8_t*)p;
-  c->op_mask = (uint32_t)(tcap - 1);
-  BXG_TRY(fill64(c, c->op_keys, tcap, kEmptyKey));
-  BXG_TRY(fill32(c, c->op_first, tcap, kNone));
-  BXG_TRY(fill32(c, c->op_last, tcap, 0));
   c->op_cap = cap;
   return BXG_OK;
 }
 
-OpScratch op_scratch(bxg_ctx* c) {
+// The scratch of a bulk call of n inputs: a table of twice the power of two
+// at or above n slots (load factor at most 1/2), emptied by the call itself.
+OpScratch op_scratch(bxg_ctx* c, uint64_t n, uint64_t* slots) {
   OpScratch os;
-  os.keys = c->op_keys;
-  os.first = c->op_first;
-  os.last = c->op_last;
-  os.mask = c->op_mask;
+  *slots = std::max<uint64_t>(pow2_at_least(n), 1 << 12) * 2;
+  os.slots = c->op_slots;
+  os.mask = (uint32_t)(*slots - 1);
   os.tpos = c->op_tpos;
+  os.leaf = c->op_leaf;
   os.prebit = c->op_prebit;
   return os;
 }
@@ -935,7 +931,10 @@ int cell_op(bxg_map* m, const int32_t* coords, const void* vals, uint64_t n, uin
   if (op != kOpSetOff)
     BXG_TRY(ensure_room(m, std::min<uint64_t>(n, 1u << 22), std::min<uint64_t>(n, 1u << 16)));
   BXG_TRY(ensure_op_scratch(c, n));
-  const OpScratch os = op_scratch(c);
+  uint64_t slots = 0;
+  const OpScratch os = op_scratch(c, n, &slots);
+  BXG_TRY(launch(c, "k_op_clear", k_op_clear, (uint32_t)std::min<uint64_t>(nblocks(slots), 4096), kBlock,
+                 os.slots, slots));
   if (op != kOpSetOff)
     BXG_TRY(launch(c, "k_op_resolve", k_op_resolve<true>, nblocks(n), kBlock, (const MapDev*)m->d,
                    coords, n, os));
@@ -944,7 +943,6 @@ int cell_op(bxg_map* m, const int32_t* coords, const void* vals, uint64_t n, uin
                    coords, n, os));
   BXG_TRY(launch(c, "k_op_apply", k_op_apply, nblocks(n), kBlock, (const MapDev*)m->d, coords,
                  (const uint8_t*)vals, n, op, was_on, os));
-  BXG_TRY(launch(c, "k_op_release", k_op_release, nblocks(n), kBlock, n, os));
   return snapshot_counters(m);
 }
 
@@ -1163,10 +1161,9 @@ int bxg_ctx_destroy(bxg_ctx* c) {
   dfree(c->d_counters);
   dfree(c->d_desc);
   dfree(c->d_hdr);
-  dfree(c->op_keys);
-  dfree(c->op_first);
-  dfree(c->op_last);
+  dfree(c->op_slots);
   dfree(c->op_tpos);
+  dfree(c->op_leaf);
   dfree(c->op_prebit);
   for (int i = 0; i < 4; ++i) dfree(c->dstage[i]);
   dfree(c->d_small);

@@ -1445,14 +1445,34 @@ __global__ void k_scan_finalize(const BatchHdr* __restrict__ hdr, const ScanDesc
 
 // ------------------------------------------------------ bulk cell ops
 
+// One slot of a bulk call's cell table: the cell's key and the first and
+// last input index naming it, on one 16-byte line so that a claim touches
+// one cache line.  An empty slot is (all ones, all ones, 0).
+struct alignas(16) OpSlot {
+  uint64_t key;
+  uint32_t first;
+  uint32_t last;
+};
+
 struct OpScratch {
-  uint64_t* keys;   // table keys (global cell index)
-  uint32_t* first;  // min input index per key
-  uint32_t* last;   // max input index per key
+  OpSlot* slots;     // the call's table (mask + 1 slots)
   uint32_t mask;
   uint32_t* tpos;    // per input: table slot (kNone if no cell)
+  uint32_t* leaf;    // per input: its leaf index, or kNone where not known yet
   uint8_t* prebit;   // per input: mask bit before the call
 };
+
+// empty the first n slots of a call's table
+__global__ void __launch_bounds__(256) k_op_clear(OpSlot* __restrict__ slots, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    OpSlot e;
+    e.key = kEmptyKey;
+    e.first = kNone;
+    e.last = 0;
+    slots[i] = e;
+  }
+}
 
 // create the root and leaf of every coordinate (setValue / setCellOn path)
 __global__ void __launch_bounds__(256) k_touch(const MapDev* __restrict__ mp,
@@ -1506,19 +1526,22 @@ __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ m
     leaf = m.children[(uint64_t)slot * m.isz + ci];
     if (leaf >= kPendingChild) return;
   }
+  // a leaf still being created by another lane: its index is looked up by
+  // k_op_apply, after this kernel
+  os.leaf[i] = leaf < kPendingChild ? leaf : kNone;
   if (leaf < kPendingChild)
     os.prebit[i] = (uint8_t)((m.masks[(uint64_t)leaf * m.lw + (li >> 6)] >> (li & 63)) & 1);
   const uint64_t key = ((uint64_t)(uint32_t)slot << 24) | ((uint64_t)ci << 12) | (uint64_t)li;
   uint32_t h = (uint32_t)mix64(key) & os.mask;
   for (uint32_t p = 0; p <= os.mask; ++p) {
-    uint64_t k = gp(os.keys)[h];
+    uint64_t k = gp(&os.slots[h])->key;
     if (k == kEmptyKey) {
-      k = cas64(&os.keys[h], kEmptyKey, key);
+      k = cas64(&os.slots[h].key, kEmptyKey, key);
       if (k == kEmptyKey) k = key;
     }
     if (k == key) {
-      min32(&os.first[h], (uint32_t)i);
-      max32(&os.last[h], (uint32_t)i);
+      min32(&os.slots[h].first, (uint32_t)i);
+      max32(&os.slots[h].last, (uint32_t)i);
       os.tpos[i] = h;
       return;
     }
@@ -1549,10 +1572,11 @@ __global__ void __launch_bounds__(256) k_op_apply(const MapDev* __restrict__ mp,
     if (was_on) was_on[i] = 0;
     return;
   }
-  const uint32_t first = os.first[t], last = os.last[t];
+  const uint32_t first = os.slots[t].first, last = os.slots[t].last;
   const uint8_t pre = os.prebit[i];
   const int32_t x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
-  const uint32_t leaf = find_leaf(m, x, y, z);
+  const uint32_t known = os.leaf[i];
+  const uint32_t leaf = known != kNone ? known : find_leaf(m, x, y, z);
   if (leaf == kNone) {
     if (was_on) was_on[i] = 0;
     return;
@@ -1587,18 +1611,6 @@ __global__ void __launch_bounds__(256) k_op_apply(const MapDev* __restrict__ mp,
   }
 }
 
-// return the table slots of the call to the empty state
-__global__ void __launch_bounds__(256) k_op_release(uint64_t n, OpScratch os) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const uint32_t t = os.tpos[i];
-  if (t == kNone) return;
-  if (os.last[t] == (uint32_t)i) {
-    os.keys[t] = kEmptyKey;
-    os.first[t] = kNone;
-    os.last[t] = 0;
-  }
-}
 
 // ConstAccessor::value / isCellOn (bonxai.hpp:496-534) for a batch.  Each
 // thread carries kLookups lookups through the chain root key -> inner entry
