@@ -48,6 +48,7 @@ METRIC = "rays/s + voxel updates/s, batched vs plain per-scan on 1 MI355X; p50/p
 
 
 LOOKUP_BYTES = 28
+INSERT_BYTES = 28
 GATHER_TABLE = 64 << 20
 
 
@@ -289,7 +290,7 @@ def run_lookups(ctx, bx, scenes, reps=20):
         g.set_values(dc, dv)
     ctx.profile(False)
     set_ms = sum(ctx.profile_read(k)[1] / max(ctx.profile_read(k)[0], 1)
-                 for k in ("k_touch", "k_op_resolve", "k_op_apply", "k_op_release"))
+                 for k in ("k_op_clear", "k_op_resolve", "k_op_apply"))
     dq = ctx.to_device(q)
     out_v = ctx.array(len(q), np.float32)
     out_f = ctx.array(len(q), np.uint8)
@@ -453,6 +454,39 @@ def latency_summary(infos, span_s):
             out[kind] = {"count": int(len(lat)), "p50_ms": float(np.percentile(lat, 50)),
                          "p99_ms": float(np.percentile(lat, 99)), "items_per_s": work / span_s}
     return out
+
+
+def run_dropin(scenes, nscans=200):
+    """The drop-in C++ API driven as the reference's ROS node drives it
+    (bonxai_amd/bin/dropin_bench): per scan of the 200-scan LiDAR sequence
+    (64 x 1024 rays, 0.1 m, 50 m), host points in a std::vector through
+    ProbabilisticMap::insertPointCloud (PCIe copy, scan, wait), then
+    getOccupiedVoxels.  None where the program was not built."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(HERE, "bonxai_amd", "bin", "dropin_bench")
+    if not os.path.exists(exe):
+        return None
+    scans = list(scenes.lidar_sequence(nscans))
+    with tempfile.NamedTemporaryFile(suffix=".bin", delete=False) as f:
+        path = f.name
+        f.write(np.array([0.1, 50.0], np.float64).tobytes())
+        f.write(np.array([len(scans)], np.uint32).tobytes())
+        for pts, origin in scans:
+            f.write(np.array([len(pts)], np.uint32).tobytes())
+            f.write(np.asarray(origin, np.float64).tobytes())
+            f.write(np.ascontiguousarray(pts, np.float32).tobytes())
+    try:
+        out = subprocess.run([exe, path], capture_output=True, text=True, timeout=300)
+    finally:
+        os.unlink(path)
+    if out.returncode != 0:
+        raise RuntimeError("dropin_bench failed: " + out.stderr.strip())
+    r = json.loads(out.stdout)
+    r["what"] = ("the C++ drop-in, as the reference's ROS node drives it: per LiDAR scan (config 3), "
+                 "ProbabilisticMap::insertPointCloud(std::vector<point>) from host memory, then "
+                 "getOccupiedVoxels; PCIe copies included")
+    return r
 
 
 def run_serving(ctx, bx, scenes, trace):
@@ -802,6 +836,13 @@ def main():
         log(f"box scan: {c1['ms']:.3f} ms ({c1['rays_per_s'] / 1e6:.0f} M rays/s), oracle {c1['cpu_ms']:.0f} ms")
         c3 = run_sequence(ctx, bx, scenes)
         log(f"lidar sequence: {c3['ms_per_scan']:.3f} ms/scan ({c3['rays_per_s'] / 1e6:.0f} M rays/s)")
+        try:
+            c3["dropin"] = run_dropin(scenes)
+        except (RuntimeError, OSError, ValueError) as e:  # reported, not fatal to the benchmark
+            c3["dropin"] = {"error": str(e)}
+        if c3["dropin"] and "error" not in c3["dropin"]:
+            log(f"drop-in C++ loop: insert p50 {c3['dropin']['insert_ms']['p50']:.3f} ms, with export "
+                f"{c3['dropin']['insert_and_export_ms']['p50']:.3f} ms")
     sv = None
     if not args.no_serving:
         sv = run_serving(ctx, bx, scenes, serving_trace(args.serve_requests, args.serve_rate))
@@ -944,6 +985,14 @@ def main():
                                             "unit": "GB/s", "frac": lk_gbs / HBM_PEAK_GBS,
                                             "traffic": measured_traffic("k_lookup"),
                                             "bytes_per_lookup": LOOKUP_BYTES}}
+        # algorithmic bytes of a setValue, the declared model: 12-byte
+        # coordinate and 4-byte value read, 4-byte cell write, 8-byte mask-word
+        # read-modify-write
+        ins_gbs = INSERT_BYTES * lk["inserts"] / (lk["insert_ms"] / 1e3) / 1e9
+        out["lookup"]["insert_roofline"] = {"bound": "hbm", "achieved": ins_gbs, "peak": HBM_PEAK_GBS,
+                                            "unit": "GB/s", "frac": ins_gbs / HBM_PEAK_GBS,
+                                            "bytes_per_insert": INSERT_BYTES,
+                                            "kernels": "k_op_clear + k_op_resolve + k_op_apply"}
     if c1 is not None:
         out["box_scan"] = c1
         out["lidar_sequence"] = c3
