@@ -1,7 +1,8 @@
 """Autotuner of the batched scan-integration launch.
 
-Sweeps, on a sample frame of uneven scans into many maps (captured-graph
-launches, as the batched form runs them):
+Sweeps, on a frame of uneven scans into many maps (captured-graph launches,
+as the batched form runs them; by default each frame into emptied maps, the
+benchmark's workload):
   ray_block           workgroup size of the two work-stealing ray walks
   ray_blocks_per_cu   workgroups per CU of the cell-update walk
   leaf_blocks_per_cu  workgroups per CU of the leaf-allocation walk
@@ -13,7 +14,7 @@ by coordinate descent from the defaults, and records the fastest setting per
 device (architecture name + CU count) in bonxai_amd/tuning.json.
 Context(tuned=True) applies the recorded setting of its device.
 
-Usage: python -m bonxai_amd.autotune [--maps 32] [--passes 2] [--out PATH]
+Usage: python -m bonxai_amd.autotune [--maps 64] [--passes 2] [--state fresh|steady] [--out PATH]
 """
 import argparse
 import json
@@ -54,35 +55,62 @@ def apply(ctx, settings):
         ctx.set_tuning(k, int(v))
 
 
-def frame_ms(ctx, bx, descs, npoints, reps=5, warmup=2):
-    """Milliseconds per frame of a graph captured with the current tuning."""
+def empty(ctx, bx, maps):
+    """Empty the maps (pools keep their capacity), as new maps are."""
+    for m in maps:
+        m.clear(bx.CLEAR_MEMORY)
+        m.update_count = 1
+    ctx.sync()
+
+
+def frame_ms(ctx, bx, descs, npoints, maps, fresh=True, reps=5, warmup=2):
+    """Milliseconds per frame of a graph captured with the current tuning:
+    each frame into emptied maps (fresh), or re-inserted into maps that hold
+    it (steady)."""
     g = bx.ScanGraph(ctx, len(descs), npoints)
     g.set_scans(descs)
     for _ in range(warmup):
+        if fresh:
+            empty(ctx, bx, maps)
         g.launch()
     ctx.sync()
-    ctx.event(0)
-    for _ in range(reps):
-        g.launch()
-    ctx.event(1)
-    ms = ctx.elapsed_ms(0, 1) / reps
+    total = 0.0
+    if fresh:
+        for _ in range(reps):
+            empty(ctx, bx, maps)
+            ctx.event(0)
+            g.launch()
+            ctx.event(1)
+            total += ctx.elapsed_ms(0, 1)
+    else:
+        ctx.event(0)
+        for _ in range(reps):
+            g.launch()
+        ctx.event(1)
+        total = ctx.elapsed_ms(0, 1)
     g.close()
-    return ms
+    for m in maps:
+        if m.sync() != 0:
+            raise RuntimeError("a map failed during tuning")
+    return total / reps
 
 
-def search(ctx, bx, scans, passes=2, log=print):
+def search(ctx, bx, scans, passes=2, fresh=True, log=print):
     max_range = 30.0
     maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
     dev = [ctx.to_device(p) for p, _ in scans]
     descs = [(m, d, o, max_range) for m, d, (_, o) in zip(maps, dev, scans)]
     npoints = sum(len(p) for p, _ in scans)
-    bx.insert_batch(ctx, descs)  # sizes the pools
-    for m in maps:
-        st = m.stats()
-        m.reserve(int(st["leaf_count"] * 1.25) + 8192, int(st["root_count"] * 2) + 1024)
+    bx.insert_batch(ctx, descs)  # sizes the pools (the library's head-room policy)
+    empty(ctx, bx, maps)
+    bx.insert_batch(ctx, descs)
     best = {k: ctx.get_tuning(k) for k in CANDIDATES}
     apply(ctx, best)
-    default_ms = best_ms = frame_ms(ctx, bx, descs, npoints)
+
+    def measure():
+        return frame_ms(ctx, bx, descs, npoints, maps, fresh)
+
+    default_ms = best_ms = measure()
     table = [(dict(best), best_ms)]
     log(f"default {best} {best_ms:.3f} ms")
     for p in range(passes):
@@ -92,7 +120,7 @@ def search(ctx, bx, scans, passes=2, log=print):
                     continue
                 trial = dict(best, **{key: v})
                 apply(ctx, trial)
-                ms = frame_ms(ctx, bx, descs, npoints)
+                ms = measure()
                 table.append((trial, ms))
                 log(f"pass {p} {key}={v}: {ms:.3f} ms")
                 if ms < best_ms * 0.99:
@@ -109,7 +137,9 @@ def search(ctx, bx, scans, passes=2, log=print):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description="autotune the batched scan launch")
-    ap.add_argument("--maps", type=int, default=32)
+    ap.add_argument("--maps", type=int, default=64)
+    ap.add_argument("--state", choices=["fresh", "steady"], default="fresh",
+                    help="time frames into emptied maps (fresh, the workload) or re-inserted ones")
     ap.add_argument("--passes", type=int, default=2)
     ap.add_argument("--out", default=TUNING_FILE)
     args = ap.parse_args(argv)
@@ -119,7 +149,7 @@ def main(argv=None):
     scans = scenes.batch_scans(nmaps=args.maps, max_range=30.0)
     ctx = bx.Context(0)
     t0 = time.time()
-    best, best_ms, default_ms, table = search(ctx, bx, scans, args.passes,
+    best, best_ms, default_ms, table = search(ctx, bx, scans, args.passes, fresh=args.state == "fresh",
                                               log=lambda *a: print(*a, flush=True))
     rec = {}
     try:
@@ -129,8 +159,8 @@ def main(argv=None):
         pass
     rec[device_key(ctx)] = {
         "settings": best, "frame_ms": best_ms, "default_frame_ms": default_ms,
-        "workload": f"{len(scans)} LiDAR scans into {len(scans)} maps, 0.1 m, 30 m, "
-                    f"{sum(len(p) for p, _ in scans)} points, captured graph",
+        "workload": f"{len(scans)} LiDAR scans into {len(scans)} {'empty' if args.state == 'fresh' else 'filled'} "
+                    f"maps, 0.1 m, 30 m, {sum(len(p) for p, _ in scans)} points, captured graph",
         "evaluations": len(table), "seconds": time.time() - t0,
     }
     with open(args.out, "w") as f:
