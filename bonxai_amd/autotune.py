@@ -10,6 +10,9 @@ benchmark's workload):
                       (rays per lane = ray_chunk / 64)
   ep_table_factor     endpoint-table slots per point (inverse load factor)
   root_slack          root-directory slots per root (inverse load factor)
+  ray_window          cells a ray may run ahead of its wavefront's slowest
+                      ray in the update pass (1 << 20: no window)
+  leaf_window         the same in leaves for the leaf pass
 by coordinate descent from the defaults, and records the fastest setting per
 device (architecture name + CU count) in bonxai_amd/tuning.json.
 Context(tuned=True) applies the recorded setting of its device.
@@ -33,6 +36,8 @@ CANDIDATES = {
     "ray_chunk": [64, 128, 256, 512],
     "ep_table_factor": [2, 4],
     "root_slack": [2, 4],
+    "ray_window": [4, 8, 16, 32, 64, 1 << 20],
+    "leaf_window": [8, 16, 32, 64, 1 << 20],
 }
 
 

@@ -107,6 +107,8 @@ struct bxg_ctx {
   uint32_t ray_chunk = 256;
   uint32_t ep_table_factor = 2;
   uint32_t root_slack = 2;
+  uint32_t ray_window = 8;    // lockstep windows of the update pass (cells) and leaf pass (leaves)
+  uint32_t leaf_window = 32;
   std::vector<Prof> prof_pending;
   std::vector<hipEvent_t> ev_free;
   std::vector<hipEvent_t> snap_pool;  // free snapshot events (SnapEvent)
@@ -720,6 +722,8 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
                           : std::max<uint32_t>(1, std::min<uint32_t>(cus * c->small_ray_blocks_per_cu, most8));
   const uint32_t gl = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->leaf_blocks_per_cu, most));
   sc.chunk_max = c->ray_chunk;
+  sc.win_cells = (int32_t)c->ray_window;
+  sc.win_leaves = (int32_t)c->leaf_window;
   // new-leaf mask pass: a wavefront per new leaf, grid-stride over them
   const uint32_t gm = std::max<uint32_t>(1, std::min<uint32_t>(cus * 8, gp));
   BXG_TRY(launch(c, "k_scan_begin", k_scan_begin, 1, kBlock, hdr, desc));
@@ -1315,6 +1319,8 @@ const TuningKey kTuning[] = {
     {"ray_chunk", &bxg_ctx::ray_chunk, 64, 1024, 64},
     {"ep_table_factor", &bxg_ctx::ep_table_factor, 2, 8, 1},
     {"root_slack", &bxg_ctx::root_slack, 2, 8, 1},
+    {"ray_window", &bxg_ctx::ray_window, 1, 1 << 20, 1},
+    {"leaf_window", &bxg_ctx::leaf_window, 1, 1 << 20, 1},
 };
 
 const TuningKey* tuning_key(const char* key) {

@@ -147,6 +147,8 @@ struct ScanScratch {
   EpSlot* ep;           // endpoint table
   uint32_t ep_mask;     // capacity - 1
   uint32_t chunk_max;   // most rays a wavefront takes per cursor atomic (64..1024)
+  int32_t win_cells;    // lockstep window of the update pass, in cells (0: the default)
+  int32_t win_leaves;   // lockstep window of the leaf pass, in leaves (0: the default)
   uint32_t* winners;    // flattened point indices of per-voxel winners, bucketed
   uint32_t* wkey;       // per point: bucket of a winner, kNone otherwise
   uint32_t* hist;       // [kMaxBuckets] winners per bucket (zero between calls)

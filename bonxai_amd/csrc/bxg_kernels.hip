@@ -1017,8 +1017,10 @@ __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const
 
 constexpr uint32_t kChunk = 256;  // default of ScanScratch::chunk_max
 // cells a lane may run ahead of its wavefront's slowest ray in lockstep
-// batches: update pass and leaf pass (measured on 64 scans a call: 8 beats
-// 4 and 16 for the update pass, 32 beats 16 for the leaf pass)
+// batches: update pass and leaf pass (defaults; the tuning keys ray_window
+// and leaf_window set them per context.  Measured on 64 scans re-inserted
+// into filled maps: 8 beats 4 and 16 for the update pass, 32 beats 16 for
+// the leaf pass)
 constexpr int32_t kSyncWindowCells = 8;
 constexpr int32_t kSyncWindowLeaves = 32;
 constexpr uint32_t kLockstepRays = 1;  // rays a lane from which batches run in lockstep
@@ -1055,6 +1057,8 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
   uint32_t chunk_pos = 0, chunk_end = 0;
   // small lists are one region: splitting them only multiplies cursor traffic
   const uint32_t cmax = sc.chunk_max ? sc.chunk_max : kChunk;
+  const int32_t win = kMode == 0 ? (sc.win_leaves > 0 ? sc.win_leaves : kSyncWindowLeaves)
+                                 : (sc.win_cells > 0 ? sc.win_cells : kSyncWindowCells);
   const uint32_t nreg = nunits >= kRegions * 8 * cmax ? kRegions : 1u;
   // chunk size: kChunk units when there is work for every wavefront, down
   // to one unit a lane on small lists
@@ -1139,7 +1143,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       // the pass at 5 waves a SIMD; the update pass measured no gain); a
       // wavefront with no lane free to move lets every lane with a ray move
       kmin = wave_min(kmin);
-      go = have && kk < kmin + kSyncWindowLeaves;
+      go = have && kk < kmin + win;
       if (__ballot(go) == 0) go = have;
     } else if (lockstep) {
 #pragma unroll
@@ -1147,7 +1151,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
         const int32_t t = __shfl_xor(kmin, o);
         kmin = t < kmin ? t : kmin;
       }
-      go = have && kk < kmin + kSyncWindowCells;
+      go = have && kk < kmin + win;
     }
     if (go) {
       if (kMode == 0) {

@@ -137,8 +137,10 @@ int bxg_set_counting(bxg_ctx* ctx, int on);
  * work-stealing update pass and leaf-allocation pass over the rays;
  * "small_ray_blocks_per_cu" (1..64, default 4), the update pass's workgroups
  * per CU for a call with fewer points than the tuned grid has lanes;
- * "ray_block", "ray_chunk", "ep_table_factor", "root_slack". Every setting
- * gives the same maps. */
+ * "ray_block", "ray_chunk", "ep_table_factor", "root_slack"; "ray_window"
+ * (default 8) and "leaf_window" (default 32), how many cells / leaves a ray
+ * may run ahead of the slowest ray of its wavefront. Every setting gives the
+ * same maps. */
 int bxg_ctx_set_tuning(bxg_ctx* ctx, const char* key, int64_t value);
 int bxg_ctx_get_tuning(bxg_ctx* ctx, const char* key, int64_t* value);
 

@@ -407,14 +407,15 @@ def test_batch_matches_sequential(ctx):
 @pytest.mark.parametrize("setting", [
     {"ray_block": 64, "ray_chunk": 64, "ray_blocks_per_cu": 16, "leaf_blocks_per_cu": 4,
      "small_ray_blocks_per_cu": 8},
-    {"ray_block": 128, "ray_chunk": 512, "ep_table_factor": 4, "root_slack": 4},
+    {"ray_block": 128, "ray_chunk": 512, "ep_table_factor": 4, "root_slack": 4, "ray_window": 1 << 20,
+     "leaf_window": 1},
     {"ray_block": 256, "ray_chunk": 1024, "ray_blocks_per_cu": 1, "leaf_blocks_per_cu": 1,
-     "small_ray_blocks_per_cu": 1},
+     "small_ray_blocks_per_cu": 1, "ray_window": 1, "leaf_window": 1 << 20},
 ])
 def test_tuning_settings_keep_results(setting):
     """Every launch setting the autotuner may pick (workgroup size, rays per
-    lane, workgroups per CU, table load factors) gives the same maps, in the
-    batched call and in a captured graph."""
+    lane, workgroups per CU, table load factors, lockstep windows) gives the
+    same maps, in the batched call and in a captured graph."""
     import bonxai_amd as bx
     from bonxai_amd import scenes
     ora = _oracle()
