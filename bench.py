@@ -289,8 +289,9 @@ def run_lookups(ctx, bx, scenes, reps=20):
         g.clear(bx.CLEAR_MEMORY)
         g.set_values(dc, dv)
     ctx.profile(False)
-    set_ms = sum(ctx.profile_read(k)[1] / max(ctx.profile_read(k)[0], 1)
-                 for k in ("k_op_clear", "k_op_resolve", "k_op_apply"))
+    set_kern = {k: ctx.profile_read(k)[1] / max(ctx.profile_read(k)[0], 1)
+                for k in ("k_op_clear", "k_op_resolve", "k_op_apply")}
+    set_ms = sum(set_kern.values())
     dq = ctx.to_device(q)
     out_v = ctx.array(len(q), np.float32)
     out_f = ctx.array(len(q), np.uint8)
@@ -340,7 +341,7 @@ def run_lookups(ctx, bx, scenes, reps=20):
         "gather_ceiling": {"reads": hits, "table_bytes": GATHER_TABLE, "ms": gather_ms,
                            "reads_per_s": hits / (gather_ms / 1e3),
                            "over_lookup_ms": gather_ms / avg_ms},
-        "inserts_per_s": len(coords) / (set_ms / 1e3),
+        "inserts_per_s": len(coords) / (set_ms / 1e3), "insert_kernels_ms": set_kern,
     }
 
 
