@@ -111,6 +111,21 @@ def check_maps(maps, what):
             raise RuntimeError(f"a map failed during the {what}")
 
 
+def timed_maps_match(sets):
+    """Whether every map of every set, as the timed frames left it, equals the
+    oracle's map of its scan: the order-free digests of the cells against
+    those committed in tests/golden/digests.json for the default frame (64
+    maps, seed 11).  None for another frame."""
+    try:
+        with open(os.path.join(HERE, "tests", "golden", "digests.json")) as f:
+            want = [w["digest"] for w in json.load(f)["maps"]["batch"]]
+    except (OSError, KeyError, ValueError):
+        return None
+    if len(want) != len(sets[0]):
+        return None
+    return all([str(x) for x in m.digest()] == want[i] for maps in sets for i, m in enumerate(maps))
+
+
 def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets, steady=True):
     """The batched form, one captured graph launch per frame of 64 scans.
     Fresh frames (the workload as stated, and `value`): each timed step
@@ -160,6 +175,7 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets, steady=True
         check_maps(ms, "timed fresh frames")
     fresh_replays = replays(sets) - replays0
     fresh_alloc = ctx.memory()[0] - mem0
+    timed_match = timed_maps_match(sets)
     # per-frame latency of fresh frames, one frame between two events
     lat_fresh = []
     for k in range(max(steps, 10)):
@@ -186,7 +202,7 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets, steady=True
     out = {
         "points": total_points, "fresh_ms": fresh_ms, "fresh_rounds": rounds, "lat_fresh": lat_fresh,
         "fresh_counts": fresh_counts, "kern_fresh": kern_fresh, "replays_timed": fresh_replays,
-        "alloc_timed": fresh_alloc, "steady_ms": None,
+        "alloc_timed": fresh_alloc, "steady_ms": None, "timed_match": timed_match,
     }
     if not steady:
         g.close()
@@ -918,6 +934,7 @@ def main():
         "frame_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
         "replays_after_timed_frames": b["replays_timed"],
         "device_bytes_allocated_by_timed_frames": b["alloc_timed"],
+        "timed_maps_equal_oracle": b["timed_match"],
         "kernels_ms": b["kern_fresh"],
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

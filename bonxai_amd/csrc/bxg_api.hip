@@ -939,11 +939,14 @@ int cell_op(bxg_map* m, const int32_t* coords, const void* vals, uint64_t n, uin
   const OpScratch os = op_scratch(c, n, &slots);
   BXG_TRY(launch(c, "k_op_clear", k_op_clear, (uint32_t)std::min<uint64_t>(nblocks(slots), 4096), kBlock,
                  os.slots, slots));
-  if (op != kOpSetOff)
-    BXG_TRY(launch(c, "k_op_resolve", k_op_resolve<true>, nblocks(n), kBlock, (const MapDev*)m->d,
+  if (op == kOpSetValue && !was_on)
+    BXG_TRY(launch(c, "k_op_resolve", k_op_resolve<true, false>, nblocks(n), kBlock, (const MapDev*)m->d,
+                   coords, n, os));
+  else if (op != kOpSetOff)
+    BXG_TRY(launch(c, "k_op_resolve", k_op_resolve<true, true>, nblocks(n), kBlock, (const MapDev*)m->d,
                    coords, n, os));
   else
-    BXG_TRY(launch(c, "k_op_resolve", k_op_resolve<false>, nblocks(n), kBlock, (const MapDev*)m->d,
+    BXG_TRY(launch(c, "k_op_resolve", k_op_resolve<false, true>, nblocks(n), kBlock, (const MapDev*)m->d,
                    coords, n, os));
   BXG_TRY(launch(c, "k_op_apply", k_op_apply, nblocks(n), kBlock, (const MapDev*)m->d, coords,
                  (const uint8_t*)vals, n, op, was_on, os));

@@ -1502,7 +1502,10 @@ __global__ void __launch_bounds__(256) k_touch(const MapDev* __restrict__ mp,
 // A cell's table key is its root slot, inner index and leaf index, known
 // before its leaf is, so a lane never waits for another lane's leaf: a leaf
 // still being created (or created in this call) has all mask bits off.
-template <bool kCreate>
+// kFirst: the call needs each cell's first input and prior mask bit (was_on
+// is asked for, or setCellOn / setCellOff); a setValue batch without was_on
+// needs only the last input of each cell (last write wins).
+template <bool kCreate, bool kFirst>
 __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ mp,
                                                     const int32_t* __restrict__ coords,
                                                     uint64_t n, OpScratch os) {
@@ -1511,7 +1514,7 @@ __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ m
   const MapView m = load_view(mp);
   const int32_t x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
   os.tpos[i] = kNone;
-  os.prebit[i] = 0;
+  if (kFirst) os.prebit[i] = 0;
   uint64_t rkey;
   if (!pack_root(x, y, z, m.log2n, &rkey)) {
     if (kCreate) or32(&m.ctr->status, kStCoordRange);
@@ -1533,7 +1536,7 @@ __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ m
   // a leaf still being created by another lane: its index is looked up by
   // k_op_apply, after this kernel
   os.leaf[i] = leaf < kPendingChild ? leaf : kNone;
-  if (leaf < kPendingChild)
+  if (kFirst && leaf < kPendingChild)
     os.prebit[i] = (uint8_t)((m.masks[(uint64_t)leaf * m.lw + (li >> 6)] >> (li & 63)) & 1);
   const uint64_t key = ((uint64_t)(uint32_t)slot << 24) | ((uint64_t)ci << 12) | (uint64_t)li;
   uint32_t h = (uint32_t)mix64(key) & os.mask;
@@ -1544,7 +1547,7 @@ __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ m
       if (k == kEmptyKey) k = key;
     }
     if (k == key) {
-      min32(&os.slots[h].first, (uint32_t)i);
+      if (kFirst) min32(&os.slots[h].first, (uint32_t)i);
       max32(&os.slots[h].last, (uint32_t)i);
       os.tpos[i] = h;
       return;
@@ -1576,8 +1579,11 @@ __global__ void __launch_bounds__(256) k_op_apply(const MapDev* __restrict__ mp,
     if (was_on) was_on[i] = 0;
     return;
   }
-  const uint32_t first = os.slots[t].first, last = os.slots[t].last;
-  const uint8_t pre = os.prebit[i];
+  const uint32_t last = os.slots[t].last;
+  // first and the prior bit exist only where k_op_resolve was asked for them
+  const bool need_first = was_on != nullptr || op != kOpSetValue;
+  const uint32_t first = need_first ? os.slots[t].first : 0u;
+  const uint8_t pre = need_first ? os.prebit[i] : (uint8_t)0;
   const int32_t x = coords[3 * i], y = coords[3 * i + 1], z = coords[3 * i + 2];
   const uint32_t known = os.leaf[i];
   const uint32_t leaf = known != kNone ? known : find_leaf(m, x, y, z);

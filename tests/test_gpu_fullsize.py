@@ -31,7 +31,8 @@ def test_clusters_full_size_against_digests(ctx):
     """2M setValue (1.49M distinct cells) and 8M lookups: was_on of every
     insert, the value and found flag of every lookup and the map's digest
     equal the oracle's (whose results were checked against the reference's
-    own VoxelGrid when the digests were written)."""
+    own VoxelGrid when the digests were written); the same inserts without
+    was_on give the same map."""
     import bonxai_amd as bx
     from bonxai_amd import scenes
     want = _golden()["maps"]["clusters"]
@@ -50,6 +51,16 @@ def test_clusters_full_size_against_digests(ctx):
     st = g.stats()
     assert st["leaf_count"] == want["leaves"] and st["root_count"] == want["roots"]
     g.close()
+    # the same inserts without was_on (the call then keeps only each cell's
+    # last input): the same map
+    g2 = bx.VoxelGrid(ctx, 0.05, dtype=np.float32)
+    dc, dv = ctx.to_device(coords), ctx.to_device(vals)
+    assert g2.set_values(dc, dv) is None
+    assert g2.sync() == 0
+    assert [str(x) for x in g2.digest()] == want["digest"]
+    dc.free()
+    dv.free()
+    g2.close()
 
 
 def test_graph_frames_full_frame_against_digests(ctx):
