@@ -190,7 +190,7 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets, steady=True
     # kernel durations, eager launches with per-kernel events
     names = ("k_scan_begin", "k_scan_classify", "k_scan_winners", "k_scan_bucket_reduce",
              "k_scan_bucket_scan", "k_scan_scatter", "k_scan_ray_leaves", "k_scan_endpoints",
-             "k_scan_rays", "k_scan_new_endpoints", "k_scan_new_masks", "k_scan_finalize")
+             "k_scan_rays", "k_scan_new_endpoints", "k_scan_new_leaves", "k_scan_finalize")
     nprof = 3
     ctx.profile_reset()
     for _ in range(nprof):

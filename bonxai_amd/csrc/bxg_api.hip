@@ -329,8 +329,13 @@ int d2h_sync(bxg_ctx* c, void* dst, const void* src, uint64_t bytes) {
 
 // ------------------------------------------------------------ map pools
 
+// bytes of touched marks a leaf (probabilistic maps: one a cell)
+uint64_t touched_bytes(const bxg_map* m) {
+  return (m->h.flags & kMapProbabilistic) ? (uint64_t)m->h.leaf_size : 0;
+}
 uint64_t leaf_bytes(const bxg_map* m) {
-  return (uint64_t)m->h.leaf_words * 8 + (uint64_t)m->h.leaf_size * m->h.value_bytes + sizeof(int4);
+  return (uint64_t)m->h.leaf_words * 8 + (uint64_t)m->h.leaf_size * m->h.value_bytes + sizeof(int4) +
+         touched_bytes(m);
 }
 uint64_t root_bytes(const bxg_map* m) { return 8 + (uint64_t)m->h.inner_size * 4; }
 
@@ -410,28 +415,31 @@ int grow_leaves(bxg_map* m, uint64_t need) {
   BXG_TRY(read_counters(m));  // also synchronizes the stream
   const uint64_t used = std::min<uint64_t>(m->h_ctr->leaf_count, m->leaf_cap);
   const uint64_t lw = m->h.leaf_words, cb = (uint64_t)m->h.leaf_size * m->h.value_bytes;
-  void *masks = nullptr, *cells = nullptr, *orig = nullptr;
+  const uint64_t tb = touched_bytes(m);
+  void *masks = nullptr, *cells = nullptr, *orig = nullptr, *touched = nullptr;
   BXG_TRY(dmalloc(c, &masks, cap * lw * 8));
   if (cb) BXG_TRY(dmalloc(c, &cells, cap * cb));
   BXG_TRY(dmalloc(c, &orig, cap * sizeof(int4)));
-  BXG_HIP(c, hipMemsetAsync(masks, 0, cap * lw * 8, c->stream));
-  if (cb) BXG_HIP(c, hipMemsetAsync(cells, 0, cap * cb, c->stream));
-  if (m->leaf_cap) {
-    BXG_HIP(c, hipMemcpyAsync(masks, m->h.leaf_masks, (uint64_t)m->leaf_cap * lw * 8,
-                              hipMemcpyDeviceToDevice, c->stream));
-    if (cb)
-      BXG_HIP(c, hipMemcpyAsync(cells, m->h.cells, (uint64_t)m->leaf_cap * cb,
-                                hipMemcpyDeviceToDevice, c->stream));
-    BXG_HIP(c, hipMemcpyAsync(orig, m->h.leaf_origin, (uint64_t)m->leaf_cap * sizeof(int4),
-                              hipMemcpyDeviceToDevice, c->stream));
+  if (tb) BXG_TRY(dmalloc(c, &touched, cap * tb));
+  // the leaves in use move over; the rest of the pool stays uninitialised
+  // (a leaf is written whole when it is created).  Touched marks start at
+  // zero, and every scan leaves them so.
+  if (tb) BXG_HIP(c, hipMemsetAsync(touched, 0, cap * tb, c->stream));
+  if (used) {
+    BXG_HIP(c, hipMemcpyAsync(masks, m->h.leaf_masks, used * lw * 8, hipMemcpyDeviceToDevice, c->stream));
+    if (cb) BXG_HIP(c, hipMemcpyAsync(cells, m->h.cells, used * cb, hipMemcpyDeviceToDevice, c->stream));
+    BXG_HIP(c, hipMemcpyAsync(orig, m->h.leaf_origin, used * sizeof(int4), hipMemcpyDeviceToDevice,
+                              c->stream));
   }
   BXG_TRY(ctx_sync(c));
   dfree(m->h.leaf_masks);
   dfree(m->h.cells);
   dfree(m->h.leaf_origin);
+  dfree(m->h.touched);
   m->h.leaf_masks = (uint64_t*)masks;
   m->h.cells = (uint8_t*)cells;
   m->h.leaf_origin = (int4*)orig;
+  m->h.touched = (uint8_t*)touched;
   m->h.leaf_cap = (uint32_t)cap;
   c->used += more;
   m->bytes += more;
@@ -724,7 +732,7 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
   sc.chunk_max = c->ray_chunk;
   sc.win_cells = (int32_t)c->ray_window;
   sc.win_leaves = (int32_t)c->leaf_window;
-  // new-leaf mask pass: a wavefront per new leaf, grid-stride over them
+  // new-leaf pass: a wavefront per new leaf, grid-stride over them
   const uint32_t gm = std::max<uint32_t>(1, std::min<uint32_t>(cus * 8, gp));
   BXG_TRY(launch(c, "k_scan_begin", k_scan_begin, 1, kBlock, hdr, desc));
   BXG_TRY(launch(c, "k_scan_classify", k_scan_classify, gp, kBlock, hdr, desc, sc));
@@ -747,7 +755,7 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
       BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<false>, nscans, kBlock, hdr, desc));
     BXG_TRY(launch(c, "k_scan_new_endpoints", k_scan_new_endpoints<false>, gw, kBlock, desc, sc));
   }
-  BXG_TRY(launch(c, "k_scan_new_masks", k_scan_new_masks, gm, kBlock, hdr, desc));
+  BXG_TRY(launch(c, "k_scan_new_leaves", k_scan_new_leaves, gm, kBlock, hdr, desc));
   BXG_TRY(launch(c, "k_scan_finalize", k_scan_finalize, 1, kBlock, hdr, desc, sc));
   return BXG_OK;
 }
@@ -939,6 +947,7 @@ int cell_op(bxg_map* m, const int32_t* coords, const void* vals, uint64_t n, uin
   const OpScratch os = op_scratch(c, n, &slots);
   BXG_TRY(launch(c, "k_op_clear", k_op_clear, (uint32_t)std::min<uint64_t>(nblocks(slots), 4096), kBlock,
                  os.slots, slots));
+  BXG_TRY(launch(c, "k_op_begin", k_op_begin, 1, 64, (const MapDev*)m->d));
   if (op == kOpSetValue && !was_on)
     BXG_TRY(launch(c, "k_op_resolve", k_op_resolve<true, false>, nblocks(n), kBlock, (const MapDev*)m->d,
                    coords, n, os));
@@ -948,6 +957,14 @@ int cell_op(bxg_map* m, const int32_t* coords, const void* vals, uint64_t n, uin
   else
     BXG_TRY(launch(c, "k_op_resolve", k_op_resolve<false, true>, nblocks(n), kBlock, (const MapDev*)m->d,
                    coords, n, os));
+  if (op != kOpSetOff) {
+    // the leaves the call created: at most one per input
+    const uint64_t words = std::min<uint64_t>(n, m->leaf_cap) *
+                           ((uint64_t)m->h.leaf_words + (uint64_t)m->h.leaf_size * m->h.value_bytes / 8);
+    const uint32_t gz = (uint32_t)std::max<uint64_t>(
+        1, std::min<uint64_t>(nblocks(words), (uint64_t)std::max(c->cu_count, 1) * 8));
+    BXG_TRY(launch(c, "k_zero_new_leaves", k_zero_new_leaves, gz, kBlock, (const MapDev*)m->d));
+  }
   BXG_TRY(launch(c, "k_op_apply", k_op_apply, nblocks(n), kBlock, (const MapDev*)m->d, coords,
                  (const uint8_t*)vals, n, op, was_on, os));
   return snapshot_counters(m);
@@ -1443,6 +1460,7 @@ int bxg_map_destroy(bxg_map* m) {
   dfree(m->h.leaf_masks);
   dfree(m->h.cells);
   dfree(m->h.leaf_origin);
+  dfree(m->h.touched);
   dfree(m->d);
   dfree(m->ctr);
   dfree(m->d_extra);
@@ -1478,15 +1496,10 @@ int bxg_map_clear(bxg_map* m, int option) {
     return ctx_sync(c);
   }
   if (option != BXG_CLEAR_MEMORY) return fail(c, BXG_ERR_INVALID_ARG, "unknown clear option");
-  // CLEAR_MEMORY: empty directory and pool (the pools keep their capacity)
+  // CLEAR_MEMORY: empty directory and pool (the pools keep their capacity;
+  // the old leaves' bytes stay: a leaf is written whole when it is created)
   BXG_TRY(fill64(c, m->h.root_keys, m->root_cap, kEmptyKey));
   BXG_TRY(fill32(c, m->h.children, (uint64_t)m->root_cap * m->h.inner_size, kEmptyChild));
-  if (nl) {
-    BXG_HIP(c, hipMemsetAsync(m->h.leaf_masks, 0, (uint64_t)nl * m->h.leaf_words * 8, c->stream));
-    if (m->h.value_bytes)
-      BXG_HIP(c, hipMemsetAsync(m->h.cells, 0, (uint64_t)nl * m->h.leaf_size * m->h.value_bytes,
-                                c->stream));
-  }
   const uint32_t z[3] = {0, 0, 0};  // leaf_count, root_count, status
   BXG_HIP(c, hipMemcpyAsync(m->ctr, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
   m->extra.clear();
@@ -1603,12 +1616,13 @@ int bxg_map_release_unused(bxg_map* m) {
   // 2. a right-sized leaf pool holding the live leaves at the front
   const uint64_t new_cap = ((uint64_t)live_leaves * 5 / 4 + 1024 + 255) & ~255ull;
   const uint64_t lw = m->h.leaf_words, cb = (uint64_t)m->h.leaf_size * m->h.value_bytes;
-  void *masks = nullptr, *cells = nullptr, *orig = nullptr;
+  const uint64_t tb = touched_bytes(m);
+  void *masks = nullptr, *cells = nullptr, *orig = nullptr, *touched = nullptr;
   BXG_TRY(dmalloc(c, &masks, new_cap * lw * 8));
   if (cb) BXG_TRY(dmalloc(c, &cells, new_cap * cb));
   BXG_TRY(dmalloc(c, &orig, new_cap * sizeof(int4)));
-  BXG_HIP(c, hipMemsetAsync(masks, 0, new_cap * lw * 8, c->stream));
-  if (cb) BXG_HIP(c, hipMemsetAsync(cells, 0, new_cap * cb, c->stream));
+  if (tb) BXG_TRY(dmalloc(c, &touched, new_cap * tb));
+  if (tb) BXG_HIP(c, hipMemsetAsync(touched, 0, new_cap * tb, c->stream));
   if (nl)
     BXG_TRY(launch(c, "k_compact_leaves", k_compact_leaves, nl, kBlock, (const MapDev*)m->d, nl,
                    (const uint32_t*)dlive, (const uint32_t*)didx, (uint64_t*)masks,
@@ -1628,10 +1642,12 @@ int bxg_map_release_unused(bxg_map* m) {
   uint64_t* old_masks = m->h.leaf_masks;
   uint8_t* old_cells = m->h.cells;
   int4* old_orig = m->h.leaf_origin;
+  uint8_t* old_touched = m->h.touched;
   const uint64_t old_bytes = (uint64_t)m->leaf_cap * leaf_bytes(m) + (uint64_t)old_rcap * root_bytes(m);
   m->h.leaf_masks = (uint64_t*)masks;
   m->h.cells = (uint8_t*)cells;
   m->h.leaf_origin = (int4*)orig;
+  m->h.touched = (uint8_t*)touched;
   m->h.leaf_cap = (uint32_t)new_cap;
   m->leaf_cap = (uint32_t)new_cap;
   m->h.root_keys = (uint64_t*)keys;
@@ -1650,6 +1666,7 @@ int bxg_map_release_unused(bxg_map* m) {
   dfree(old_masks);
   dfree(old_cells);
   dfree(old_orig);
+  dfree(old_touched);
   const uint64_t new_bytes = new_cap * leaf_bytes(m) + rcap * root_bytes(m);
   c->used = c->used - old_bytes + new_bytes;
   m->bytes = m->bytes - old_bytes + new_bytes;

@@ -14,12 +14,20 @@
 //                    word n >> 6, bit n & 63, as mask.hpp:329-365)
 //                    cells[(leaf * leaf_size + n) * value_bytes]
 //                    leaf_origin[leaf] = min-corner coordinate of the leaf.
+//                    touched[leaf * leaf_size + n] (probabilistic maps): one
+//                    byte a cell, nonzero only while a scan runs, for the
+//                    cells that scan updates in the leaves it creates.
 //
 // Invariant kept by every kernel: a cell whose mask bit is off holds all-zero
 // bytes.  The reference never reads an off cell's storage (value() returns
 // null, setValue/setCellOn/value(create) overwrite it, forEachCell and
 // Serialize skip it), so the invariant is unobservable, and it lets the
 // scan-integration kernels derive "cell was off" from "cell word is zero".
+// Pool entries at or past leaf_count hold undefined bytes (a cleared or grown
+// pool is not zeroed): whatever creates a leaf writes all its cells and mask
+// words, as the reference's Grid constructor initialises a new leaf
+// (bonxai.hpp:51-58): the scan's leaf assembly (k_scan_new_leaves), the bulk
+// calls' k_zero_new_leaves, or k_write_leaves.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -56,8 +64,8 @@ struct alignas(64) MapCounters {
   uint64_t stat_rays;
   uint64_t stat_ray_updates;
   uint64_t stat_ray_visits;  // counted runs: cells the scan's rays crossed
-  // leaf_count when the running scan started: leaves at or above it were
-  // created by that scan, so all their cells were off (zero) before it
+  // leaf_count when the running scan or bulk call started: leaves at or
+  // above it were created by that call, and it initialises them
   uint32_t leaf_base;
   uint32_t pad;
 };
@@ -75,6 +83,7 @@ struct alignas(64) MapDev {
   uint64_t* leaf_masks;
   uint8_t* cells;
   int4* leaf_origin;
+  uint8_t* touched;  // leaf_cap * leaf_size bytes, probabilistic maps only
   uint32_t leaf_cap;
   int32_t opt_miss, opt_hit, opt_cmin, opt_cmax, opt_thr;
   MapCounters* ctr;

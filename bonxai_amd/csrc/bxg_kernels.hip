@@ -157,6 +157,7 @@ struct MapView {
   uint32_t BXG_G* children;
   uint64_t BXG_G* masks;
   uint8_t BXG_G* cells;
+  uint8_t BXG_G* touched;
   int4* origin;  // HIP vector types keep the generic address space
   MapCounters* ctr;
   double inv_res;
@@ -172,6 +173,7 @@ __device__ __forceinline__ MapView load_view(const MapDev* p) {
   v.children = gp(q->children);
   v.masks = gp(q->leaf_masks);
   v.cells = gp(q->cells);
+  v.touched = gp(q->touched);
   v.origin = q->leaf_origin;
   v.ctr = q->ctr;
   v.inv_res = q->inv_res;
@@ -602,15 +604,14 @@ __device__ __forceinline__ uint32_t walk_leaf(const MapView& m, const RayWalk& w
 // leaf of the next run (looked up before this run's cells are loaded, so the
 // two memory round trips overlap).
 //
-// Leaves at or above `lbase` were created by the running scan, so each of
-// their cells held zero (off) before it.  The ray update of such a cell gives
-// the same word whichever ray gets there first, cell_miss(0), and the only
-// cells of these leaves that must not end with it are endpoints of the scan,
-// which k_scan_new_endpoints writes after this pass.  So the runs in new
-// leaves are stored without being read, and their mask words are built after
-// the pass from the nonzero cells (k_scan_new_masks), with no atomic a cell.
-// In older leaves each cell is read, updated unless its update_id is cur, and
-// a cell turned on sets its mask bit with an atomicOr.
+// Leaves at or above `lbase` were created by the running scan: each of their
+// cells was off (zero) before it, and the ray update of such a cell gives the
+// same word whichever ray gets there first, cell_miss(0).  So a run in a new
+// leaf only marks its cells in the leaf's touched bytes (plain byte stores of
+// 1, every writer agreeing; no read, no atomic), and k_scan_new_leaves writes
+// the leaf's cells and mask words from those bytes after the scan's endpoints
+// are marked.  In older leaves each cell is read, updated unless its
+// update_id is cur, and a cell turned on sets its mask bit with an atomicOr.
 template <bool kCount>
 __device__ __forceinline__ uint32_t apply_step(const MapView& m, RayWalk& w, RootCache& rc,
                                                int32_t cur, uint32_t lbase, uint32_t& next_leaf) {
@@ -619,21 +620,24 @@ __device__ __forceinline__ uint32_t apply_step(const MapView& m, RayWalk& w, Roo
   const uint32_t n = collect_run(m, w, li);
   next_leaf = w.done() ? kNone : walk_leaf(m, w, rc);
   if (leaf == kNone) return 0;
-  uint32_t BXG_G* base = (uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz;
   if (leaf >= lbase) {
-    const uint32_t fresh = cell_miss(0u, m.miss, m.cmin, cur);
+    uint8_t BXG_G* tb = m.touched + (uint64_t)leaf * m.lsz;
     uint32_t nupd = 0;
 #pragma unroll
     for (int j = 0; j < kRun; ++j) {
       if ((uint32_t)j < n) {
-        if (kCount)
-          nupd += cas32((uint32_t*)&base[li[j]], 0u, fresh) == 0u ? 1u : 0u;
-        else
-          base[li[j]] = fresh;
+        if (kCount) {
+          // exact first marks: an atomic or on the byte's aligned word
+          const uint32_t sh = (li[j] & 3u) * 8u;
+          nupd += ((or32((uint32_t*)(tb + (li[j] & ~3u)), 1u << sh) >> sh) & 0xFFu) == 0u ? 1u : 0u;
+        } else {
+          tb[li[j]] = 1;
+        }
       }
     }
     return nupd;
   }
+  uint32_t BXG_G* base = (uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz;
   uint32_t v[kRun];
 #pragma unroll
   for (int j = 0; j < kRun; ++j)
@@ -1257,10 +1261,12 @@ __global__ void __launch_bounds__(256) k_scan_endpoints(const ScanDesc* __restri
 // Phase 4d (per winner): the endpoint update of the endpoints in leaves the
 // scan created (k_scan_endpoints left their leaf index in tpos).  Such a
 // cell held zero before the scan, so its endpoint word is cell_hit(0) or
-// cell_miss(0), whatever the ray pass stored there meanwhile (the reference
-// updates endpoints first and its rays skip them, probabilistic_map.cpp:30-54,
-// 81-89).  The counted form takes back the ray update the ray pass counted
-// for a cell that turns out to be an endpoint.
+// cell_miss(0), whatever ray marked it meanwhile (the reference updates
+// endpoints first and its rays skip them, probabilistic_map.cpp:30-54,
+// 81-89): its touched byte becomes 2 (hit) or 1 (miss), one writer per
+// voxel, and k_scan_new_leaves writes the word.  The counted form takes back
+// the ray update the ray pass counted for a cell that turns out to be an
+// endpoint.
 template <bool kCount>
 __global__ void __launch_bounds__(256) k_scan_new_endpoints(const ScanDesc* __restrict__ sd,
                                                             ScanScratch sc) {
@@ -1277,12 +1283,9 @@ __global__ void __launch_bounds__(256) k_scan_new_endpoints(const ScanDesc* __re
       if (leaf != kNone) {
         const int4 r = sc.rec[gi];
         const MapView m = load_view(sd[(uint32_t)r.w >> 16].map);
-        const int32_t cur = map_cur(m);
-        uint32_t BXG_G* cell =
-            (uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz + leaf_index(r.x, r.y, r.z, m.lb);
-        const uint32_t word = (r.w & kRecHit) ? cell_hit(0u, m.hit, m.cmax, cur) : cell_miss(0u, m.miss, m.cmin, cur);
-        if (kCount) retracted = *cell != 0u ? 1u : 0u;
-        *cell = word;
+        uint8_t BXG_G* tb = m.touched + (uint64_t)leaf * m.lsz + leaf_index(r.x, r.y, r.z, m.lb);
+        if (kCount) retracted = *tb != 0u ? 1u : 0u;
+        *tb = (r.w & kRecHit) ? 2 : 1;
         updated = 1;
         ctr = m.ctr;
       }
@@ -1340,15 +1343,20 @@ __global__ void __launch_bounds__(256) k_scan_extra_rays(const BatchHdr* __restr
   }
 }
 
-// Phase 4c: the mask words of the leaves the scan created.  Their cells were
-// all zero (off) before it, and every cell the scan updated holds a nonzero
-// word (update_id = cur, 1..3), so such a leaf's mask is exactly its nonzero
-// cells: one ballot per 64 cells, in place of an atomicOr per cell turned on
-// (Accessor::value(create) sets the bit, bonxai.hpp:485-490).  One wavefront
-// per new leaf; the new leaves of each scan's map are [leaf_base,
-// leaf_count).  Maps whose call failed wrote no cell and are skipped.
-__global__ void __launch_bounds__(256) k_scan_new_masks(const BatchHdr* __restrict__ hdr,
-                                                        const ScanDesc* __restrict__ sd) {
+// Phase 4c: the leaves the scan created, written whole.  Their cells were
+// all off before the scan (a created leaf is initialised, as the reference's
+// Grid constructor does, bonxai.hpp:51-58), and the scan left in each
+// cell's touched byte what it did there: 0 nothing, 1 a miss (a ray, or a
+// miss endpoint: cell_miss(0)), 2 a hit endpoint (cell_hit(0)).  So every
+// cell word follows from its byte, and the mask is exactly the nonzero bytes
+// (Accessor::value(create) sets the bit, bonxai.hpp:485-490): one pass per
+// new leaf writes its cells and mask words and clears the bytes it read,
+// with no read of the cell pool and no atomic.  The new leaves of each
+// scan's map are [leaf_base, leaf_count); a map whose call failed wrote no
+// byte, and its new leaves are written off (zero).
+constexpr uint32_t kNewLeafRun = 4;  // leaves a wavefront of k_scan_new_leaves has in flight
+__global__ void __launch_bounds__(256) k_scan_new_leaves(const BatchHdr* __restrict__ hdr,
+                                                         const ScanDesc* __restrict__ sd) {
   const uint32_t lane = lane_id();
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
@@ -1356,30 +1364,78 @@ __global__ void __launch_bounds__(256) k_scan_new_masks(const BatchHdr* __restri
   for (uint32_t s = 0; s < nscans; ++s) {
     const MapView m = load_view(sd[s].map);
     const MapCounters BXG_G* ctr = gp(m.ctr);
-    if (ctr->status & kFatal) continue;
     const uint32_t lo = ctr->leaf_base;
     const uint32_t hi = min(ctr->leaf_count, m.leaf_cap);
-    if (m.lw == 8) {  // 8^3 leaves (the default leaf_bits 3): eight loads in flight
-      for (uint32_t leaf = lo + wave; leaf < hi; leaf += nwaves) {
-        const uint32_t BXG_G* cells = (const uint32_t BXG_G*)m.cells + (uint64_t)leaf * 512u;
-        uint32_t v[8];
+    if (lo >= hi) continue;
+    const int32_t cur = ctr->cur;
+    const uint32_t wmiss = cell_miss(0u, m.miss, m.cmin, cur);
+    const uint32_t whit = cell_hit(0u, m.hit, m.cmax, cur);
+    if (m.lsz == 512) {
+      // 8^3 leaves (the default leaf_bits 3): lane l has cells 4l..4l+3 and
+      // 256+4l..256+4l+3, so each store instruction writes 1 KB in one piece
+      // (measured: 2.98 ms a fresh frame's new leaves, against 3.63 ms with
+      // 8 consecutive cells a lane).  The cell stores are nontemporal: a
+      // frame's new leaves (8.4 GB) would only push its other data out of
+      // the caches (2.52 ms against 2.98 ms).
+      typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+      for (uint32_t leaf0 = lo + wave; leaf0 < hi; leaf0 += kNewLeafRun * nwaves) {
+        uint32_t t[kNewLeafRun][2];
 #pragma unroll
-        for (uint32_t w = 0; w < 8; ++w) v[w] = cells[w * 64u + lane];
-        uint64_t b[8];
+        for (uint32_t k = 0; k < kNewLeafRun; ++k) {
+          const uint32_t leaf = leaf0 + k * nwaves;
+          const uint32_t BXG_G* tp = (const uint32_t BXG_G*)(m.touched + (uint64_t)leaf * 512u) + lane;
+          t[k][0] = leaf < hi ? tp[0] : 0u;
+          t[k][1] = leaf < hi ? tp[64] : 0u;
+        }
 #pragma unroll
-        for (uint32_t w = 0; w < 8; ++w) b[w] = __ballot(v[w] != 0u);
-        // lane w stores word w: one 64-byte write per leaf
-        uint64_t mine = b[0];
+        for (uint32_t k = 0; k < kNewLeafRun; ++k) {
+          const uint32_t leaf = leaf0 + k * nwaves;
+          if (leaf >= hi) break;  // uniform across the wavefront
 #pragma unroll
-        for (uint32_t w = 1; w < 8; ++w) mine = lane == w ? b[w] : mine;
-        if (lane < 8) m.masks[(uint64_t)leaf * 8u + lane] = mine;
+          for (uint32_t h = 0; h < 2; ++h) {
+            uint32_t v[4];
+            uint32_t nib = 0;
+#pragma unroll
+            for (uint32_t b = 0; b < 4; ++b) {
+              const uint32_t cls = (t[k][h] >> (8 * b)) & 0xFFu;
+              v[b] = cls == 0u ? 0u : (cls == 1u ? wmiss : whit);
+              nib |= (cls != 0u ? 1u : 0u) << b;
+            }
+            u32x4 BXG_G* cp = (u32x4 BXG_G*)(m.cells + (uint64_t)leaf * 2048u + 1024u * h) + lane;
+            __builtin_nontemporal_store(u32x4{v[0], v[1], v[2], v[3]}, cp);
+            if (t[k][h]) ((uint32_t BXG_G*)(m.touched + (uint64_t)leaf * 512u + 256u * h))[lane] = 0u;
+            // mask word 4h + r: the nibbles of lanes 16r..16r+15, lane 16r + i
+            // at bits 4i..4i+3; lanes 16r..16r+7 fill the low half, 16r+8..
+            // 16r+15 the high half.  OR over each half-row of 8 lanes (DPP
+            // quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror), then lane 16r
+            // takes the high half from lane 16r+8 (row_ror:8).
+            const uint32_t sh = 4u * (lane & 7u);
+            uint32_t mlo = (lane & 8u) ? 0u : nib << sh;
+            uint32_t mhi = (lane & 8u) ? nib << sh : 0u;
+            mlo |= (uint32_t)__builtin_amdgcn_update_dpp((int)mlo, (int)mlo, 0xB1, 0xF, 0xF, false);
+            mhi |= (uint32_t)__builtin_amdgcn_update_dpp((int)mhi, (int)mhi, 0xB1, 0xF, 0xF, false);
+            mlo |= (uint32_t)__builtin_amdgcn_update_dpp((int)mlo, (int)mlo, 0x4E, 0xF, 0xF, false);
+            mhi |= (uint32_t)__builtin_amdgcn_update_dpp((int)mhi, (int)mhi, 0x4E, 0xF, 0xF, false);
+            mlo |= (uint32_t)__builtin_amdgcn_update_dpp((int)mlo, (int)mlo, 0x141, 0xF, 0xF, false);
+            mhi |= (uint32_t)__builtin_amdgcn_update_dpp((int)mhi, (int)mhi, 0x141, 0xF, 0xF, false);
+            mhi = (uint32_t)__builtin_amdgcn_update_dpp((int)mhi, (int)mhi, 0x128, 0xF, 0xF, false);
+            if ((lane & 15u) == 0u)
+              m.masks[(uint64_t)leaf * 8u + 4u * h + (lane >> 4)] = ((uint64_t)mhi << 32) | mlo;
+          }
+        }
       }
     } else {
       for (uint32_t leaf = lo + wave; leaf < hi; leaf += nwaves) {
-        const uint32_t BXG_G* cells = (const uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz;
+        uint8_t BXG_G* tb = m.touched + (uint64_t)leaf * m.lsz;
+        uint32_t BXG_G* cells = (uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz;
         for (uint32_t w = 0; w < m.lw; ++w) {
           const uint32_t i = w * 64u + lane;
-          const uint64_t b = __ballot((i < m.lsz ? cells[i] : 0u) != 0u);
+          const uint32_t cls = i < m.lsz ? tb[i] : 0u;
+          if (i < m.lsz) {
+            cells[i] = cls == 0u ? 0u : (cls == 1u ? wmiss : whit);
+            if (cls) tb[i] = 0;
+          }
+          const uint64_t b = __ballot(cls != 0u);
           if (lane == 0) m.masks[(uint64_t)leaf * m.lw + w] = b;
         }
       }
@@ -1478,6 +1534,35 @@ __global__ void __launch_bounds__(256) k_op_clear(OpSlot* __restrict__ slots, ui
   }
 }
 
+// The leaf count when a bulk call starts: the leaves at or above it are the
+// call's, created with undefined bytes.
+__global__ void k_op_begin(const MapDev* __restrict__ mp) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const MapView m = load_view(mp);
+  MapCounters BXG_G* ctr = gp(m.ctr);
+  ctr->leaf_base = min(ctr->leaf_count, m.leaf_cap);
+}
+
+// The leaves a bulk call created, initialised before any cell of them is
+// written: mask words and cell bytes zero (every cell off), as the
+// reference's Grid constructor leaves a new leaf (bonxai.hpp:51-58).  Grid-
+// stride over 8-byte words (a leaf's cells are a multiple of 8 bytes).
+__global__ void __launch_bounds__(256) k_zero_new_leaves(const MapDev* __restrict__ mp) {
+  const MapView m = load_view(mp);
+  const MapCounters BXG_G* ctr = gp(m.ctr);
+  const uint64_t lo = ctr->leaf_base;
+  const uint64_t hi = min(ctr->leaf_count, m.leaf_cap);
+  if (lo >= hi) return;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const uint64_t start = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t nm = (hi - lo) * m.lw;
+  for (uint64_t i = start; i < nm; i += stride) m.masks[lo * m.lw + i] = 0;
+  const uint64_t cw = (uint64_t)m.lsz * m.vb / 8;  // 8-byte words a leaf
+  uint64_t BXG_G* cells = (uint64_t BXG_G*)m.cells + lo * cw;
+  const uint64_t nc = (hi - lo) * cw;
+  for (uint64_t i = start; i < nc; i += stride) cells[i] = 0;
+}
+
 // create the root and leaf of every coordinate (setValue / setCellOn path)
 __global__ void __launch_bounds__(256) k_touch(const MapDev* __restrict__ mp,
                                                const int32_t* __restrict__ coords, uint64_t n) {
@@ -1501,7 +1586,8 @@ __global__ void __launch_bounds__(256) k_touch(const MapDev* __restrict__ mp,
 // setValue / setCellOn; setCellOff creates nothing and skips absent cells).
 // A cell's table key is its root slot, inner index and leaf index, known
 // before its leaf is, so a lane never waits for another lane's leaf: a leaf
-// still being created (or created in this call) has all mask bits off.
+// still being created (or created in this call: index at or above the
+// call's leaf_base, its bytes not yet initialised) has all mask bits off.
 // kFirst: the call needs each cell's first input and prior mask bit (was_on
 // is asked for, or setCellOn / setCellOff); a setValue batch without was_on
 // needs only the last input of each cell (last write wins).
@@ -1536,7 +1622,7 @@ __global__ void __launch_bounds__(256) k_op_resolve(const MapDev* __restrict__ m
   // a leaf still being created by another lane: its index is looked up by
   // k_op_apply, after this kernel
   os.leaf[i] = leaf < kPendingChild ? leaf : kNone;
-  if (kFirst && leaf < kPendingChild)
+  if (kFirst && leaf < *gp(&m.ctr->leaf_base))
     os.prebit[i] = (uint8_t)((m.masks[(uint64_t)leaf * m.lw + (li >> 6)] >> (li & 63)) & 1);
   const uint64_t key = ((uint64_t)(uint32_t)slot << 24) | ((uint64_t)ci << 12) | (uint64_t)li;
   uint32_t h = (uint32_t)mix64(key) & os.mask;
