@@ -190,7 +190,8 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets, steady=True
     # kernel durations, eager launches with per-kernel events
     names = ("k_scan_begin", "k_scan_classify", "k_scan_winners", "k_scan_bucket_reduce",
              "k_scan_bucket_scan", "k_scan_scatter", "k_scan_ray_leaves", "k_scan_endpoints",
-             "k_scan_rays", "k_scan_new_endpoints", "k_scan_new_leaves", "k_scan_finalize")
+             "k_scan_fresh_rays", "k_scan_rays", "k_scan_new_endpoints", "k_scan_new_leaves",
+             "k_scan_finalize")
     nprof = 3
     ctx.profile_reset()
     for _ in range(nprof):
@@ -797,6 +798,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=1500.0, help="offered requests per second")
     ap.add_argument("--ray-blocks-per-cu", type=int, default=0,
                     help="override the tuned workgroups per CU of the ray kernels")
+    ap.add_argument("--tuning", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a launch setting of the context (repeatable), after the tuned ones")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run the plain copy kernel over MIB MiB five times (the known-size "
                          "reference for scaling HBM byte counters) and exit")
@@ -823,6 +826,9 @@ def main():
     ctx = bx.Context(0, tuned=True)
     if args.ray_blocks_per_cu:
         ctx.set_tuning("ray_blocks_per_cu", args.ray_blocks_per_cu)
+    for kv in args.tuning:
+        k, v = kv.split("=", 1)
+        ctx.set_tuning(k, int(v))
     dev = [ctx.to_device(p) for p, _ in scans]
 
     t0 = time.time()

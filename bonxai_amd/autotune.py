@@ -13,6 +13,8 @@ benchmark's workload):
   ray_window          cells a ray may run ahead of its wavefront's slowest
                       ray in the update pass (1 << 20: no window)
   leaf_window         the same in leaves for the leaf pass
+  fresh_pass          maps empty at a scan's start: one create-and-mark walk
+                      (1) or the leaf and update walks (0)
 by coordinate descent from the defaults, and records the fastest setting per
 device (architecture name + CU count) in bonxai_amd/tuning.json.
 Context(tuned=True) applies the recorded setting of its device.
@@ -38,6 +40,7 @@ CANDIDATES = {
     "root_slack": [2, 4],
     "ray_window": [4, 8, 16, 32, 64, 1 << 20],
     "leaf_window": [8, 16, 32, 64, 1 << 20],
+    "fresh_pass": [1, 0],
 }
 
 

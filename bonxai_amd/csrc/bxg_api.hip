@@ -109,6 +109,7 @@ struct bxg_ctx {
   uint32_t root_slack = 2;
   uint32_t ray_window = 8;    // lockstep windows of the update pass (cells) and leaf pass (leaves)
   uint32_t leaf_window = 32;
+  uint32_t fresh_pass = 1;    // maps empty at a scan's start: one create-and-mark walk
   std::vector<Prof> prof_pending;
   std::vector<hipEvent_t> ev_free;
   std::vector<hipEvent_t> snap_pool;  // free snapshot events (SnapEvent)
@@ -732,9 +733,10 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
   sc.chunk_max = c->ray_chunk;
   sc.win_cells = (int32_t)c->ray_window;
   sc.win_leaves = (int32_t)c->leaf_window;
+  sc.fresh_pass = c->fresh_pass;
   // new-leaf pass: a wavefront per new leaf, grid-stride over them
   const uint32_t gm = std::max<uint32_t>(1, std::min<uint32_t>(cus * 8, gp));
-  BXG_TRY(launch(c, "k_scan_begin", k_scan_begin, 1, kBlock, hdr, desc));
+  BXG_TRY(launch(c, "k_scan_begin", k_scan_begin, 1, kBlock, hdr, desc, sc));
   BXG_TRY(launch(c, "k_scan_classify", k_scan_classify, gp, kBlock, hdr, desc, sc));
   BXG_TRY(launch(c, "k_scan_winners", k_scan_winners, gp, kBlock, hdr, desc, sc));
   BXG_TRY(launch(c, "k_scan_bucket_reduce", k_scan_bucket_reduce, kTiles, 1024, hdr, sc));
@@ -745,11 +747,13 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
     BXG_TRY(launch(c, "k_scan_extra_leaves", k_scan_extra_leaves, nscans, kBlock, hdr, desc));
   BXG_TRY(launch(c, "k_scan_endpoints", k_scan_endpoints, gw, kBlock, desc, sc));
   if (c->counting) {
+    BXG_TRY(launch(c, "k_scan_fresh_rays", k_scan_fresh_rays<true>, gr, rb, desc, sc));
     BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<true>, gr, rb, desc, sc));
     if (any_extra)
       BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<true>, nscans, kBlock, hdr, desc));
     BXG_TRY(launch(c, "k_scan_new_endpoints", k_scan_new_endpoints<true>, gw, kBlock, desc, sc));
   } else {
+    BXG_TRY(launch(c, "k_scan_fresh_rays", k_scan_fresh_rays<false>, gr, rb, desc, sc));
     BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<false>, gr, rb, desc, sc));
     if (any_extra)
       BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<false>, nscans, kBlock, hdr, desc));
@@ -1341,6 +1345,7 @@ const TuningKey kTuning[] = {
     {"root_slack", &bxg_ctx::root_slack, 2, 8, 1},
     {"ray_window", &bxg_ctx::ray_window, 1, 1 << 20, 1},
     {"leaf_window", &bxg_ctx::leaf_window, 1, 1 << 20, 1},
+    {"fresh_pass", &bxg_ctx::fresh_pass, 0, 1, 1},
 };
 
 const TuningKey* tuning_key(const char* key) {

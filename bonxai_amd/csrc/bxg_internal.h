@@ -132,11 +132,19 @@ constexpr uint32_t kRegions = 8;   // winner-list regions, one per XCD
 constexpr uint32_t kScanTile = 4096;  // buckets one block of the bucket scan covers
 constexpr uint32_t kTiles = kMaxBuckets / kScanTile;
 static_assert((1u << kDirBits) % kScanTile == 0 && kTiles <= 64, "bucket scan tiles");
-// scan counter words: [0] winner count, [1, 1 + 4 * kRegions) region cursors
-// and exhausted flags of the two walks, [kTileWords, + kTiles) bucket-tile sums
+// scan counter words: [0] winner count, [1, 1 + 6 * kRegions) region cursors
+// and exhausted flags of the three walks (leaf pass, update pass, fresh
+// pass), [kFreshWords, + 4) the fresh-pass split (below), [kTileWords, +
+// kTiles) bucket-tile sums
 constexpr uint32_t kTileWords = 64;
 constexpr uint32_t kCounterWords = kTileWords + kTiles;
-static_assert(1 + 4 * kRegions <= kTileWords, "counter words");
+// With at most 2^kMaxTagBits scans (one bucket tag each), the scans whose map
+// is empty at the start take the highest tags, so their winners end the
+// list: [kFreshWords, +1] the 64-bit mask of those scans, [+2] the first of
+// their tags (kNone: no split, every pass walks the whole list and skips the
+// other pipeline's rays), [+3] the index of their first winner.
+constexpr uint32_t kFreshWords = 1 + 6 * kRegions;
+static_assert(kFreshWords + 4 <= kTileWords, "counter words");
 
 // scratch of one integration call, owned by the context
 // One slot of the per-call endpoint table: key (scan tag, root slot, inner
@@ -158,12 +166,14 @@ struct ScanScratch {
   uint32_t chunk_max;   // most rays a wavefront takes per cursor atomic (64..1024)
   int32_t win_cells;    // lockstep window of the update pass, in cells (0: the default)
   int32_t win_leaves;   // lockstep window of the leaf pass, in leaves (0: the default)
+  uint32_t fresh_pass;  // 1: the rays of maps empty at the scan's start take the fresh pass
   uint32_t* winners;    // flattened point indices of per-voxel winners, bucketed
   uint32_t* wkey;       // per point: bucket of a winner, kNone otherwise
   uint32_t* hist;       // [kMaxBuckets] winners per bucket (zero between calls)
   uint32_t* offs;       // [kMaxBuckets] bucket write cursors
   uint32_t* counters;   // [0] winner count, [1..16] leaf-pass region cursors
-                        // and exhausted flags, [17..32] the same for the ray pass
+                        // and exhausted flags, [17..32] the same for the ray
+                        // pass, [33..48] for the fresh pass
 };
 
 // rec.w flag bits

@@ -668,6 +668,82 @@ __device__ __forceinline__ uint32_t apply_step(const MapView& m, RayWalk& w, Roo
   return nupd;
 }
 
+// The fresh pass: rays of a map that was empty when the scan started, so
+// every leaf they cross is new.  One walk creates the leaves (as the leaf
+// pass) and marks the ray's cells (as the update pass does in new leaves),
+// where the two-pass pipeline walks each ray twice.  Nothing it writes is
+// visible in the map's cells until k_scan_new_leaves, and a failed call's
+// marks are dropped there, so the pass may stop at an overflow like the leaf
+// pass.  Per lane: the root slot and inner-grid entry of the current cell's
+// leaf, loaded one step ahead.
+
+// root slot (created if missing) and inner-grid entry of the walk's current
+// cell (a plain load: a stale EMPTY only sends the lane to the claim's CAS, a
+// stale PENDING to an atomic re-read)
+__device__ __forceinline__ void fresh_entry(const MapView& m, const RayWalk& w, RootCache& rc,
+                                            int32_t& slot, uint32_t& entry) {
+  slot = root_slot<true>(m, rc, w.cx, w.cy, w.cz);
+  entry = slot >= 0
+              ? *gp(&m.children[(uint64_t)slot * m.isz + inner_index(w.cx, w.cy, w.cz, m.lb, m.ib)])
+              : kEmptyChild;
+}
+
+// One leaf of a fresh-pass ray: the leaf is created if missing (the lowest
+// lane of each run of lanes at one entry claims it, one aggregated bump per
+// wavefront), then the run's cells before the endpoint are marked in the
+// leaf's touched bytes.  A leaf another wavefront is still creating
+// (PENDING) is re-read and the step taken again on the lane's next turn; its
+// creator publishes it within its own step.  A full root directory or pool
+// (the call fails) stops the ray.  Returns the cells marked first (counted
+// form, exact, by atomics).
+template <bool kCount>
+__device__ __forceinline__ uint32_t fresh_step(const MapView& m, RayWalk& w, RootCache& rc,
+                                               int32_t& slot, uint32_t& entry, bool& stopped) {
+  uint32_t* cp = slot >= 0 ? (uint32_t*)&m.children[(uint64_t)slot * m.isz +
+                                                    inner_index(w.cx, w.cy, w.cz, m.lb, m.ib)]
+                           : nullptr;
+  const bool missing = slot >= 0 && entry == kEmptyChild;
+  const int lead = run_leader(missing, (uint64_t)cp, 0u);
+  bool won = false;
+  uint32_t seen = entry;
+  if (missing && lead == (int)lane_id()) won = leaf_claim(cp, seen);
+  const int32_t lmask = ~((1 << m.lb) - 1);
+  const uint32_t idx = leaf_publish(m, won, cp, (uint32_t)slot, w.cx & lmask, w.cy & lmask, w.cz & lmask);
+  if (won) seen = idx;
+  // the other lanes of a leader's run take its outcome
+  const uint32_t led = (uint32_t)__shfl((int)seen, lead);
+  if (missing) seen = led;
+  if (slot < 0 || seen == kEmptyChild) {
+    stopped = true;
+    return 0;
+  }
+  if (seen == kPendingChild) {
+    entry = ld32(cp);
+    return 0;
+  }
+  const int32_t k0 = w.k;
+  uint32_t li[kRun];
+  const uint32_t n = collect_run(m, w, li);
+  // cells k0 .. k0 + n - 1 of the walk; the ray's are those before mx
+  const int32_t rest = w.mx - k0;
+  const uint32_t nray = rest <= 0 ? 0u : (n < (uint32_t)rest ? n : (uint32_t)rest);
+  uint8_t BXG_G* tb = m.touched + (uint64_t)seen * m.lsz;
+  uint32_t nupd = 0;
+#pragma unroll
+  for (int j = 0; j < kRun; ++j) {
+    if ((uint32_t)j < nray) {
+      if (kCount) {
+        const uint32_t sh = (li[j] & 3u) * 8u;
+        nupd += ((or32((uint32_t*)(tb + (li[j] & ~3u)), 1u << sh) >> sh) & 0xFFu) == 0u ? 1u : 0u;
+      } else {
+        tb[li[j]] = 1;
+      }
+    }
+  }
+  if (!w.done()) fresh_entry(m, w, rc, slot, entry);
+  return nupd;
+}
+
 // --------------------------------------------------------- scan phases
 
 __device__ __forceinline__ uint32_t find_scan(const ScanDesc* __restrict__ sd, uint32_t nscans,
@@ -883,7 +959,17 @@ __global__ void __launch_bounds__(256) k_scan_winners(const BatchHdr* __restrict
       win = true;
       const uint32_t s = (uint32_t)r.w >> 16;
       const ScanDesc& d = sd[s];
-      const uint32_t tag = s & ((1u << hdr->tag_bits) - 1u);
+      uint32_t tag = s & ((1u << hdr->tag_bits) - 1u);
+      const uint32_t split = sc.counters[kFreshWords + 2];
+      if (split != kNone) {
+        // the other scans keep their order in tags [0, split), the fresh
+        // scans theirs in [split, nscans)
+        const uint64_t fresh = (uint64_t)sc.counters[kFreshWords] |
+                               ((uint64_t)sc.counters[kFreshWords + 1] << 32);
+        const uint64_t below = (1ull << s) - 1ull;
+        tag = ((fresh >> s) & 1ull) ? split + (uint32_t)__popcll(fresh & below)
+                                    : (uint32_t)__popcll(~fresh & below);
+      }
       key = (tag << kDirBits) | dir_bin(r.x - d.ox, r.y - d.oy, r.z - d.oz);
     }
     sc.wkey[gi] = win ? key : kNone;
@@ -959,7 +1045,14 @@ __global__ void __launch_bounds__(1024) k_scan_bucket_scan(const BatchHdr* __res
   o.w = run;
   reinterpret_cast<uint4*>(sc.offs + base)[threadIdx.x] = o;
   h4[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
-  if (base + kScanTile >= nb && threadIdx.x == 1023) sc.counters[0] = carry + wsum[15];
+  // the fresh scans' winners start at bucket split << kDirBits (a tile's
+  // first bucket: kScanTile divides 2^kDirBits); with no fresh scan, at the end
+  const uint32_t split = sc.counters[kFreshWords + 2];
+  if (split != kNone && (split << kDirBits) == base && threadIdx.x == 0) sc.counters[kFreshWords + 3] = o.x;
+  if (base + kScanTile >= nb && threadIdx.x == 1023) {
+    sc.counters[0] = carry + wsum[15];
+    if (split != kNone && (split << kDirBits) >= nb) sc.counters[kFreshWords + 3] = carry + wsum[15];
+  }
 }
 
 // Phase 2t (per point): winners written into their buckets.
@@ -973,8 +1066,15 @@ __global__ void __launch_bounds__(256) k_scan_scatter(const BatchHdr* __restrict
   if (win) sc.winners[pos] = (uint32_t)gi;
 }
 
+// The passes over the winners' rays (walk_winners, start_ray).
+enum : int { kPassLeaves = 0, kPassUpdate = 1, kPassFresh = 2 };
+
 // Start the ray of winner `idx`; false if it has no cells to walk (origin
-// voxel endpoint, endpoint-only call, or a map whose call failed).
+// voxel endpoint, endpoint-only call, or a map whose call failed), or if the
+// ray belongs to the other pipeline: with sc.fresh_pass set, the rays of a
+// map that was empty when the scan started (leaf_base 0: every leaf the scan
+// touches is new) take only the fresh pass, and the other maps' rays only the
+// leaf and update passes.
 // With lg > 0, work unit `unit` is segment unit & (2^lg - 1) of winner
 // unit >> lg: segment s of n = 2^lg walks cells [ceil(s mx / n), ceil((s + 1)
 // mx / n)), so segment 0 holds cell 0 and the segments cover the ray once.
@@ -982,7 +1082,10 @@ __global__ void __launch_bounds__(256) k_scan_scatter(const BatchHdr* __restrict
 __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const ScanScratch& sc,
                                           uint32_t unit, uint32_t lg, MapView& m, RayWalk& w,
                                           RootCache& rc, int32_t& cur, uint32_t& lbase,
-                                          bool leaf_pass, bool& first) {
+                                          int pass, bool& first) {
+  // the leaf and fresh passes walk the endpoint's cell too (they create
+  // leaves); the update pass only the ray's cells
+  const bool leaf_pass = pass != kPassUpdate;
   const uint32_t idx = unit >> lg, seg = unit & ((1u << lg) - 1u);
   first = seg == 0;
   const uint32_t gi = sc.winners[idx];
@@ -997,9 +1100,10 @@ __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const
   if (!leaf_pass && (no_rays || at_origin)) return false;
   if (leaf_pass && no_rays && seg != 0) return false;
   m = load_view(d.map);
-  if (!leaf_pass && (map_status(m) & kFatal)) return false;
+  if (pass != kPassLeaves && (map_status(m) & kFatal)) return false;
   cur = map_cur(m);
   lbase = *gp(&m.ctr->leaf_base);
+  if (sc.fresh_pass ? ((pass == kPassFresh) != (lbase == 0)) : pass == kPassFresh) return false;
   w.init(d.ox, d.oy, d.oz, r.x, r.y, r.z);
   const int32_t last = w.mx + (leaf_pass ? 1 : 0);
   int32_t k0 = 0, k1 = last;
@@ -1043,13 +1147,22 @@ constexpr uint32_t kMaxSegLog = 3;     // at most 8 segments a ray when rays are
 // call) lanes refill one by one: there, a batch waiting for its longest ray
 // leaves the GPU idle.  A range found empty
 // is flagged so that idle wavefronts stop touching its cursor.  kMode 0: leaf
-// allocation pass; 1: update pass; 2: counted update.
+// allocation pass; 1: update pass; 2: counted update; 3: fresh pass (maps
+// empty at the scan's start: create and mark in one walk); 4: counted fresh
+// pass.
 // cursors[0..kRegions) are the cursors, cursors[kRegions..2*kRegions) the
 // exhausted flags.
 template <int kMode>
 __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch& sc,
                              uint32_t* cursors) {
-  const uint32_t nwin = sc.counters[0];
+  if (kMode >= 3 && !sc.fresh_pass) return;
+  // the pass's part of the winner list: with a split, the fresh scans'
+  // winners end the list
+  const uint32_t total = sc.counters[0];
+  const bool split = sc.counters[kFreshWords + 2] != kNone;
+  const uint32_t at = split ? sc.counters[kFreshWords + 3] : 0u;
+  const uint32_t w0 = (split && kMode >= 3) ? at : 0u;
+  const uint32_t nwin = split ? (kMode >= 3 ? total - at : at) : total;
   const uint32_t lane = lane_id();
   const uint32_t lanes = gridDim.x * blockDim.x;
   const bool lockstep = nwin >= kLockstepRays * lanes;
@@ -1061,6 +1174,9 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
   uint32_t chunk_pos = 0, chunk_end = 0;
   // small lists are one region: splitting them only multiplies cursor traffic
   const uint32_t cmax = sc.chunk_max ? sc.chunk_max : kChunk;
+  constexpr bool kCounted = kMode == 2 || kMode == 4;
+  constexpr bool kFreshPass = kMode >= 3;
+  constexpr int kPass = kMode == 0 ? kPassLeaves : (kFreshPass ? kPassFresh : kPassUpdate);
   const int32_t win = kMode == 0 ? (sc.win_leaves > 0 ? sc.win_leaves : kSyncWindowLeaves)
                                  : (sc.win_cells > 0 ? sc.win_cells : kSyncWindowCells);
   const uint32_t nreg = nunits >= kRegions * 8 * cmax ? kRegions : 1u;
@@ -1080,6 +1196,11 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
   uint32_t lbase = 0;
   uint32_t nupd = 0;
   uint32_t next_leaf = kNone;
+  // fresh pass: root slot and inner-grid entry of the current cell's leaf
+  int32_t slot = -1;
+  uint32_t entry = kEmptyChild;
+  bool stopped = false;
+  bool ray_first = false;
   // counted form: per-lane totals, flushed when the lane's map changes
   MapCounters* cmap = nullptr;
   uint64_t c_upd = 0, c_rays = 0, c_vis = 0;
@@ -1117,11 +1238,16 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       const uint32_t take = want < avail ? want : avail;
       const uint32_t rank = (uint32_t)__popcll(need & lanes_below());
       if (!have && rank < take) {
-        have = start_ray(sd, sc, chunk_pos + rank, lg, m, w, rc, cur, lbase, kMode == 0, first);
+        have = start_ray(sd, sc, chunk_pos + rank + (w0 << lg), lg, m, w, rc, cur, lbase, kPass, first);
         nupd = 0;
+        stopped = false;
+        // the unit holds cell 0 of a ray with cells (the fresh pass also
+        // walks origin-voxel endpoints and endpoint-only calls: no ray)
+        ray_first = have && first && w.k < w.mx;
         if (kMode == 0 && have) lw.init(w, m.lb);
-        if (kMode != 0 && have) next_leaf = walk_leaf(m, w, rc);
-        if (kMode == 2 && have && m.ctr != cmap) {
+        if ((kMode == 1 || kMode == 2) && have) next_leaf = walk_leaf(m, w, rc);
+        if (kFreshPass && have) fresh_entry(m, w, rc, slot, entry);
+        if (kCounted && have && m.ctr != cmap) {
           if (cmap) {
             add64(&cmap->stat_ray_updates, c_upd);
             add64(&cmap->stat_rays, c_rays);
@@ -1131,6 +1257,8 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
           c_upd = c_rays = c_vis = 0;
         }
         if (kMode == 2 && have) c_vis += (uint32_t)(w.kend - w.k);
+        // the fresh pass also walks the endpoint's cell mx, not a ray cell
+        if (kMode == 4 && have) c_vis += (uint32_t)max(0, min(w.kend, w.mx) - w.k);
       }
       chunk_pos += take;
     }
@@ -1160,19 +1288,21 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
     if (go) {
       if (kMode == 0) {
         leaves_step(m, lw, rc);
+      } else if (kFreshPass) {
+        nupd += fresh_step<kCounted>(m, w, rc, slot, entry, stopped);
       } else {
-        nupd += apply_step<kMode == 2>(m, w, rc, cur, lbase, next_leaf);
+        nupd += apply_step<kCounted>(m, w, rc, cur, lbase, next_leaf);
       }
-      if (kMode == 0 ? lw.done() : w.done()) {
+      if (kMode == 0 ? lw.done() : (w.done() || stopped)) {
         have = false;
-        if (kMode == 2) {
+        if (kCounted) {
           c_upd += nupd;
-          c_rays += first ? 1u : 0u;
+          c_rays += ray_first ? 1u : 0u;
         }
       }
     }
   }
-  if (kMode == 2 && cmap) {
+  if (kCounted && cmap) {
     add64(&cmap->stat_ray_updates, c_upd);
     add64(&cmap->stat_rays, c_rays);
     add64(&cmap->stat_ray_visits, c_vis);
@@ -1207,6 +1337,8 @@ __global__ void __launch_bounds__(256) k_scan_extra_leaves(const BatchHdr* __res
   }
 }
 
+constexpr uint32_t kLeafLater = 0xFFFFFFFEu;  // tpos: look the endpoint's leaf up after the fresh pass
+
 // Phase 3 (per winner): endpoint hit/miss update (probabilistic_map.cpp:30-54)
 // of the endpoints in leaves that existed before the scan, and release of
 // the endpoint-table slot.  One writer per endpoint cell.  An endpoint in a
@@ -1232,9 +1364,13 @@ __global__ void __launch_bounds__(256) k_scan_endpoints(const ScanDesc* __restri
       e->first = kNone;
       const ScanDesc& d = sd[(uint32_t)r.w >> 16];
       const MapView m = load_view(d.map);
-      const uint32_t leaf = (map_status(m) & kFatal) ? kNone : find_leaf(m, r.x, r.y, r.z);
-      const bool fresh = leaf != kNone && leaf >= *gp(&m.ctr->leaf_base);
-      sc.tpos[gi] = fresh ? leaf : kNone;
+      const bool failed = (map_status(m) & kFatal) != 0;
+      const uint32_t leaf = failed ? kNone : find_leaf(m, r.x, r.y, r.z);
+      const uint32_t lbase = *gp(&m.ctr->leaf_base);
+      const bool fresh = leaf != kNone && leaf >= lbase;
+      // in a map empty at the scan's start the fresh pass, which runs after
+      // this kernel, creates the endpoint's leaf: k_scan_new_endpoints finds it
+      sc.tpos[gi] = fresh ? leaf : (leaf == kNone && lbase == 0 && !failed ? kLeafLater : kNone);
       if (leaf != kNone && !fresh) {
         const uint32_t li = leaf_index(r.x, r.y, r.z, m.lb);
         uint32_t BXG_G* cell = (uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz + li;
@@ -1259,7 +1395,8 @@ __global__ void __launch_bounds__(256) k_scan_endpoints(const ScanDesc* __restri
 }
 
 // Phase 4d (per winner): the endpoint update of the endpoints in leaves the
-// scan created (k_scan_endpoints left their leaf index in tpos).  Such a
+// scan created (k_scan_endpoints left their leaf index in tpos, or
+// kLeafLater where the fresh pass was still to create it).  Such a
 // cell held zero before the scan, so its endpoint word is cell_hit(0) or
 // cell_miss(0), whatever ray marked it meanwhile (the reference updates
 // endpoints first and its rays skip them, probabilistic_map.cpp:30-54,
@@ -1279,10 +1416,11 @@ __global__ void __launch_bounds__(256) k_scan_new_endpoints(const ScanDesc* __re
     MapCounters* ctr = nullptr;
     if (idx < nwin) {
       const uint32_t gi = sc.winners[idx];
-      const uint32_t leaf = sc.tpos[gi];
+      uint32_t leaf = sc.tpos[gi];
+      const int4 r = sc.rec[gi];
+      const MapView m = load_view(sd[(uint32_t)r.w >> 16].map);
+      if (leaf == kLeafLater) leaf = (map_status(m) & kFatal) ? kNone : find_leaf(m, r.x, r.y, r.z);
       if (leaf != kNone) {
-        const int4 r = sc.rec[gi];
-        const MapView m = load_view(sd[(uint32_t)r.w >> 16].map);
         uint8_t BXG_G* tb = m.touched + (uint64_t)leaf * m.lsz + leaf_index(r.x, r.y, r.z, m.lb);
         if (kCount) retracted = *tb != 0u ? 1u : 0u;
         *tb = (r.w & kRecHit) ? 2 : 1;
@@ -1312,6 +1450,15 @@ template <bool kCount>
 __global__ void __launch_bounds__(256) k_scan_rays(const ScanDesc* __restrict__ sd,
                                                    ScanScratch sc) {
   walk_winners<kCount ? 2 : 1>(sd, sc, &sc.counters[1 + 2 * kRegions]);
+}
+
+// Phase 4' (per winner of a map empty at the scan's start, with
+// ScanScratch::fresh_pass): leaf creation and the ray clear in one walk (the
+// leaf pass and the update pass skip these rays).
+template <bool kCount>
+__global__ void __launch_bounds__(256) k_scan_fresh_rays(const ScanDesc* __restrict__ sd,
+                                                         ScanScratch sc) {
+  walk_winners<kCount ? 4 : 3>(sd, sc, &sc.counters[1 + 4 * kRegions]);
 }
 
 // Phase 4b (one block per scan): rays due from addHitPoint / addMissPoint.
@@ -1368,8 +1515,11 @@ __global__ void __launch_bounds__(256) k_scan_new_leaves(const BatchHdr* __restr
     const uint32_t hi = min(ctr->leaf_count, m.leaf_cap);
     if (lo >= hi) continue;
     const int32_t cur = ctr->cur;
-    const uint32_t wmiss = cell_miss(0u, m.miss, m.cmin, cur);
-    const uint32_t whit = cell_hit(0u, m.hit, m.cmax, cur);
+    // a failed call's leaves stay off: its marks (the fresh pass may stop
+    // part-way) are cleared and give no cell
+    const bool failed = (ctr->status & kFatal) != 0;
+    const uint32_t wmiss = failed ? 0u : cell_miss(0u, m.miss, m.cmin, cur);
+    const uint32_t whit = failed ? 0u : cell_hit(0u, m.hit, m.cmax, cur);
     if (m.lsz == 512) {
       // 8^3 leaves (the default leaf_bits 3): lane l has cells 4l..4l+3 and
       // 256+4l..256+4l+3, so each store instruction writes 1 KB in one piece
@@ -1399,7 +1549,7 @@ __global__ void __launch_bounds__(256) k_scan_new_leaves(const BatchHdr* __restr
             for (uint32_t b = 0; b < 4; ++b) {
               const uint32_t cls = (t[k][h] >> (8 * b)) & 0xFFu;
               v[b] = cls == 0u ? 0u : (cls == 1u ? wmiss : whit);
-              nib |= (cls != 0u ? 1u : 0u) << b;
+              nib |= (cls != 0u && !failed ? 1u : 0u) << b;
             }
             u32x4 BXG_G* cp = (u32x4 BXG_G*)(m.cells + (uint64_t)leaf * 2048u + 1024u * h) + lane;
             __builtin_nontemporal_store(u32x4{v[0], v[1], v[2], v[3]}, cp);
@@ -1435,7 +1585,7 @@ __global__ void __launch_bounds__(256) k_scan_new_leaves(const BatchHdr* __restr
             cells[i] = cls == 0u ? 0u : (cls == 1u ? wmiss : whit);
             if (cls) tb[i] = 0;
           }
-          const uint64_t b = __ballot(cls != 0u);
+          const uint64_t b = __ballot(cls != 0u && !failed);
           if (lane == 0) m.masks[(uint64_t)leaf * m.lw + w] = b;
         }
       }
@@ -1470,12 +1620,27 @@ __global__ void k_na_recover(MapCounters* __restrict__ ctr, uint32_t leaf_cap) {
 
 // Phase 0 (per scan): the leaf count at the start of the scan, so that later
 // phases can tell the leaves the scan creates from older ones.
-__global__ void k_scan_begin(const BatchHdr* __restrict__ hdr, const ScanDesc* __restrict__ sd) {
-  for (uint32_t s = threadIdx.x; s < hdr->nscans; s += blockDim.x) {
+// With the fresh pass, also the scans whose map is empty (their winners get
+// the highest bucket tags: the fresh pass walks the end of the list, the
+// other passes the rest).
+__global__ void k_scan_begin(const BatchHdr* __restrict__ hdr, const ScanDesc* __restrict__ sd,
+                             ScanScratch sc) {
+  const uint32_t nscans = hdr->nscans;
+  for (uint32_t s = threadIdx.x; s < nscans; s += blockDim.x) {
     const MapView m = load_view(sd[s].map);
     MapCounters BXG_G* ctr = gp(m.ctr);
-    ctr->leaf_base = min(ctr->leaf_count, m.leaf_cap);
+    const uint32_t base = min(ctr->leaf_count, m.leaf_cap);
+    ctr->leaf_base = base;
+    if (nscans <= 64u) {  // the first wavefront holds every scan
+      const uint64_t fresh = __ballot(sc.fresh_pass && base == 0);
+      if (threadIdx.x == 0) {
+        sc.counters[kFreshWords] = (uint32_t)fresh;
+        sc.counters[kFreshWords + 1] = (uint32_t)(fresh >> 32);
+        sc.counters[kFreshWords + 2] = sc.fresh_pass ? nscans - (uint32_t)__popcll(fresh) : kNone;
+      }
+    }
   }
+  if ((nscans > 64u || nscans == 0u) && threadIdx.x == 0) sc.counters[kFreshWords + 2] = kNone;
 }
 
 // Phase 5 (per scan): advance update_count 1 -> 2 -> 3 -> 1
@@ -1500,7 +1665,7 @@ __global__ void k_scan_finalize(const BatchHdr* __restrict__ hdr, const ScanDesc
 #pragma unroll
     for (uint32_t k = 0; k < sizeof(MapCounters) / 8; ++k) dst[k] = v[k];
   }
-  if (threadIdx.x <= 4 * kRegions) sc.counters[threadIdx.x] = 0;
+  if (threadIdx.x <= 6 * kRegions) sc.counters[threadIdx.x] = 0;
 }
 
 // ------------------------------------------------------ bulk cell ops

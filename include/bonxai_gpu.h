@@ -139,8 +139,10 @@ int bxg_set_counting(bxg_ctx* ctx, int on);
  * per CU for a call with fewer points than the tuned grid has lanes;
  * "ray_block", "ray_chunk", "ep_table_factor", "root_slack"; "ray_window"
  * (default 8) and "leaf_window" (default 32), how many cells / leaves a ray
- * may run ahead of the slowest ray of its wavefront. Every setting gives the
- * same maps. */
+ * may run ahead of the slowest ray of its wavefront; "fresh_pass" (default
+ * 1): the rays of a map that is empty when its scan starts create their
+ * leaves and mark their cells in one walk (0: the leaf pass and the update
+ * pass, as for other maps). Every setting gives the same maps. */
 int bxg_ctx_set_tuning(bxg_ctx* ctx, const char* key, int64_t value);
 int bxg_ctx_get_tuning(bxg_ctx* ctx, const char* key, int64_t* value);
 

@@ -404,6 +404,51 @@ def test_batch_matches_sequential(ctx):
         d.free()
 
 
+def test_batch_mixed_empty_and_filled_maps(ctx):
+    """One batch whose maps alternate between empty ones (their rays take the
+    fresh pass, whose winners end the list) and ones that already hold a
+    scan (leaf and update passes), eager and through a captured graph, also
+    counted: every map and its ray counts equal the oracle's."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    scans = scenes.batch_scans(nmaps=7, lo=3_000, hi=40_000, seed=21)
+    prior = scenes.batch_scans(nmaps=7, lo=3_000, hi=20_000, seed=22)
+    filled = [i % 2 == 1 for i in range(len(scans))]
+    for use_graph in (False, True):
+        maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+        models = [ora.ProbMap(0.1) for _ in scans]
+        for m, om, f, (p, o) in zip(maps, models, filled, prior):
+            if f:
+                m.insert_point_cloud(p, o, 30.0)
+                om.insert_point_cloud(p, o, 30.0)
+        dev = [ctx.to_device(p) for p, _ in scans]
+        descs = [(m, d, o, 30.0) for m, d, (_, o) in zip(maps, dev, scans)]
+        ctx.counting(True)
+        if use_graph:
+            g = bx.ScanGraph(ctx, len(scans), sum(len(p) for p, _ in scans))
+            g.set_scans(descs)
+            g.launch()
+            ctx.sync()
+            g.close()
+        else:
+            bx.insert_batch(ctx, descs)
+        ctx.counting(False)
+        for i, (m, om, (p, o)) in enumerate(zip(maps, models, scans)):
+            assert m.sync() == 0
+            om.insert_point_cloud(p, o, 30.0)
+            what = f"map {i} ({'filled' if filled[i] else 'empty'}, graph {use_graph})"
+            assert_same_cells(_gpu_cells(m), om.cells(), what)
+            st, ost = m.stats(), om.stats()
+            assert st["last_rays"] == ost["rays"] and st["last_ray_updates"] == ost["ray_updates"], what
+            assert st["last_ray_visits"] == ost["visits"], what
+            assert m.update_count == om.update_count
+            om.close()
+            m.close()
+        for d in dev:
+            d.free()
+
+
 @pytest.mark.parametrize("setting", [
     {"ray_block": 64, "ray_chunk": 64, "ray_blocks_per_cu": 16, "leaf_blocks_per_cu": 4,
      "small_ray_blocks_per_cu": 8},
@@ -411,11 +456,14 @@ def test_batch_matches_sequential(ctx):
      "leaf_window": 1},
     {"ray_block": 256, "ray_chunk": 1024, "ray_blocks_per_cu": 1, "leaf_blocks_per_cu": 1,
      "small_ray_blocks_per_cu": 1, "ray_window": 1, "leaf_window": 1 << 20},
+    {"fresh_pass": 0},
+    {"fresh_pass": 1, "ray_window": 1, "ray_block": 64},
 ])
 def test_tuning_settings_keep_results(setting):
     """Every launch setting the autotuner may pick (workgroup size, rays per
-    lane, workgroups per CU, table load factors, lockstep windows) gives the
-    same maps, in the batched call and in a captured graph."""
+    lane, workgroups per CU, table load factors, lockstep windows, the fresh
+    pass on or off for maps empty at a scan's start) gives the same maps, in
+    the batched call and in a captured graph."""
     import bonxai_amd as bx
     from bonxai_amd import scenes
     ora = _oracle()
