@@ -18,6 +18,7 @@
 // Pointers read out of a map header are cast to the global address space
 // (BXG_G) so loads compile to global_load rather than flat_load.
 #include "bxg_internal.h"
+#include "bxg_walk.h"
 
 #define BXG_G __attribute__((address_space(1)))
 
@@ -361,100 +362,10 @@ __device__ __forceinline__ uint32_t cell_miss(uint32_t w, int32_t miss, int32_t 
 
 // ------------------------------------------------------------- rays
 
-// Integer 3-D Bresenham walk of RayIterator (probabilistic_map.hpp:162-203):
-// cell 0 is the origin voxel, then max|d|-1 further cells; the end voxel is
-// not part of the ray and a zero-length ray is empty.  (cx, cy, cz) is cell k.
-struct RayWalk {
-  int32_t cx, cy, cz;
-  int32_t sx, sy, sz;
-  int32_t dx, dy, dz;
-  int32_t ex, ey, ez;
-  int32_t mx, k, kend;
-  __device__ __forceinline__ void init(int32_t ox, int32_t oy, int32_t oz, int32_t tx, int32_t ty,
-                                       int32_t tz) {
-    cx = ox;
-    cy = oy;
-    cz = oz;
-    dx = tx - ox;
-    dy = ty - oy;
-    dz = tz - oz;
-    sx = dx < 0 ? -1 : 1;
-    sy = dy < 0 ? -1 : 1;
-    sz = dz < 0 ? -1 : 1;
-    dx = dx < 0 ? -dx : dx;
-    dy = dy < 0 ? -dy : dy;
-    dz = dz < 0 ? -dz : dz;
-    mx = dx > dy ? dx : dy;
-    mx = mx > dz ? mx : dz;
-    ex = ey = ez = 0;
-    k = 0;
-    kend = mx;
-  }
-  // walk only cells [k0, k1) of the ray: cell k0 and its error terms by the
-  // closed form (n steps along an axis after k cells: n = floor((2k|d| +
-  // mx) / (2mx)), error k|d| - n mx), the same state stepping from 0 reaches
-  __device__ __forceinline__ void seek(int32_t k0, int32_t k1) {
-    const int64_t m2 = 2 * (int64_t)mx;
-    const int64_t nx = (2 * (int64_t)k0 * dx + mx) / m2;
-    const int64_t ny = (2 * (int64_t)k0 * dy + mx) / m2;
-    const int64_t nz = (2 * (int64_t)k0 * dz + mx) / m2;
-    cx += sx * (int32_t)nx;
-    cy += sy * (int32_t)ny;
-    cz += sz * (int32_t)nz;
-    ex = (int32_t)((int64_t)k0 * dx - nx * mx);
-    ey = (int32_t)((int64_t)k0 * dy - ny * mx);
-    ez = (int32_t)((int64_t)k0 * dz - nz * mx);
-    k = k0;
-    kend = k1;
-  }
-  __device__ __forceinline__ bool done() const { return k >= kend; }
-  // move to cell k + 1; h = ceil(mx / 2), so that 2 e >= mx <=> e >= h
-  // (one compare an axis instead of a shift and a compare)
-  __device__ __forceinline__ void advance(int32_t h) {
-    ++k;
-    ex += dx;
-    ey += dy;
-    ez += dz;
-    if (ex >= h) {
-      cx += sx;
-      ex -= mx;
-    }
-    if (ey >= h) {
-      cy += sy;
-      ey -= mx;
-    }
-    if (ez >= h) {
-      cz += sz;
-      ez -= mx;
-    }
-  }
-};
-
-constexpr int kRun = 8;  // cells of one leaf handled per step of a ray
-
-// Advance a walk through the cells of its current leaf (at most kRun of
-// them), recording their leaf-local indices; returns how many.  Slots past
-// the run hold the run's first index, so that every slot names a cell of the
-// leaf and its load needs no branch (it reads the first cell's line again).
+// The cell walk of a ray (RayWalk, collect_cells) is in bxg_walk.h, shared
+// with the host check of the walk.
 __device__ __forceinline__ uint32_t collect_run(const MapView& m, RayWalk& w, uint32_t li[kRun]) {
-  const int32_t lmask = ~((1 << m.lb) - 1);
-  const int32_t lx = w.cx & lmask, ly = w.cy & lmask, lz = w.cz & lmask;
-  const int32_t h = (w.mx + 1) >> 1;
-  const uint32_t l0 = leaf_index(w.cx, w.cy, w.cz, m.lb);
-  uint32_t n = 0;
-#pragma unroll
-  for (int j = 0; j < kRun; ++j) {
-    li[j] = l0;
-    // one test without short-circuit branches: not done and still in the
-    // leaf (once false it stays false, since the walk no longer moves)
-    const bool in = (w.k < w.kend) & ((((w.cx ^ lx) | (w.cy ^ ly) | (w.cz ^ lz)) & lmask) == 0);
-    if (in) {
-      li[j] = leaf_index(w.cx, w.cy, w.cz, m.lb);
-      n = j + 1;
-      w.advance(h);
-    }
-  }
-  return n;
+  return collect_cells(w, m.lb, li);
 }
 
 // The leaves a ray's cells [k, kend) lie in, one leaf per step, without
