@@ -15,7 +15,9 @@ hold it is reported beside it as "steady".
 
 Prints one JSON line.  Extra fields: the plain per-scan rate, the
 batched/plain ratio, voxel updates/s, per-frame p50/p99 latency, the
-roofline of the ray kernel (k_scan_rays) and of the bulk lookup and insert
+roofline of the dominant scan kernel (the fresh pass, k_scan_fresh_rays, on
+empty maps; k_scan_rays on steady frames), of the leaf assembly, and of the
+bulk lookup and insert
 kernels (VoxelGrid<float>, 2M inserts / 8M lookups), the serving form (16
 resident maps, a Poisson stream of depth-frame inserts, lookups and exports;
 p50/p99 per request type), and the CPU baselines.
@@ -887,15 +889,33 @@ def main():
     # voxel updates per frame: endpoint updates (one per traced ray) + ray
     # updates (every distinct cell a ray crossed whose update_id was not cur)
     upd = fc["rays"] + fc["ray_updates"]
-    # algorithmic bytes of one k_scan_rays launch: per ray its 16-byte record
-    # and 4-byte list entry; per updated cell one 4-byte read and one 4-byte
-    # write (every distinct cell once)
-    k_ms = b["kern_fresh"]["k_scan_rays"]
-    alg_bytes = 20 * fc["rays"] + 8 * fc["ray_updates"]
+    # the dominant kernel of a fresh frame: the fresh pass (maps empty at the
+    # scan's start), or the update pass where the fresh pass is switched off.
+    # Algorithmic bytes of one fresh-pass launch: per ray its 16-byte record
+    # and 4-byte list entry, per updated cell its 1-byte mark, per new leaf
+    # its 16-byte origin and 4-byte inner-grid entry.  Of one update-pass
+    # launch: per ray 20 bytes, per updated cell one 4-byte read and one
+    # 4-byte write (every distinct cell once).
+    kf = b["kern_fresh"]
+    fresh_pass = kf.get("k_scan_fresh_rays", 0.0) > kf["k_scan_rays"]
+    if fresh_pass:
+        dom = "k_scan_fresh_rays"
+        alg_bytes = 20 * fc["rays"] + 1 * fc["ray_updates"] + 20 * fc["leaves"]
+    else:
+        dom = "k_scan_rays"
+        alg_bytes = 20 * fc["rays"] + 8 * fc["ray_updates"]
+    k_ms = kf[dom]
     achieved = alg_bytes / (k_ms / 1e3) / 1e9
+    # the leaf assembly: per new leaf 512 mark bytes read, 2048 cell bytes
+    # and 64 mask bytes written
+    nl_ms = kf.get("k_scan_new_leaves", 0.0)
+    nl_bytes = 2624 * fc["leaves"]
     # the whole frame on the integration model: 12 B a point read, 8 B per
-    # updated cell, 64 B of mask words per new leaf
+    # updated cell, 64 B of mask words per new leaf; counting the leaves'
+    # own initialisation (in a fresh frame every updated cell is in a new
+    # leaf, which is written whole): 12 B a point, 2112 B a new leaf
     frame_bytes = 12 * npts + 8 * upd + 64 * fc["leaves"]
+    frame_bytes_init = 12 * npts + (2048 + 64) * fc["leaves"]
     from bonxai_amd import autotune
     ctx_tuning = {k: ctx.get_tuning(k) for k in autotune.CANDIDATES}
     cpu_rate, cpu_maps, cpu_pts, cpu_dt = cpu_baseline(scans, max_range, args.cpu_seconds)
@@ -944,14 +964,22 @@ def main():
         "kernels_ms": b["kern_fresh"],
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_scan_rays"),
-            "kernel": "k_scan_rays",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom),
+            "kernel": dom,
             "alg_bytes_per_launch": alg_bytes,
+        },
+        "new_leaves_roofline": None if nl_ms <= 0 else {
+            "bound": "hbm", "kernel": "k_scan_new_leaves", "alg_bytes_per_launch": nl_bytes,
+            "achieved": nl_bytes / (nl_ms / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": nl_bytes / (nl_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": measured_traffic("k_scan_new_leaves"),
         },
         "frame_roofline": {
             "bound": "hbm", "alg_bytes_per_frame": frame_bytes,
             "achieved": frame_bytes / (ms_step / 1e3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": frame_bytes / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "alg_bytes_per_frame_with_leaf_init": frame_bytes_init,
+            "frac_with_leaf_init": frame_bytes_init / (ms_step / 1e3) / 1e9 / HBM_PEAK_GBS,
         },
         "cpu_baseline": {
             "value": cpu_rate, "unit": "rays/s", "cores": 1, "kind": "port",

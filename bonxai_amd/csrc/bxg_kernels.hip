@@ -141,8 +141,13 @@ __device__ __forceinline__ uint32_t wave_bump(uint32_t* counter, bool pred) {
 // where `on` is set.
 __device__ __forceinline__ int run_leader(bool on, uint64_t k1, uint32_t k2) {
   const uint32_t lane = lane_id();
-  const uint64_t p1 = __shfl_up(k1, 1u);
-  const uint32_t p2 = __shfl_up(k2, 1u);
+  // the previous lane's keys: DPP wave_shr:1 (lane i reads lane i - 1; lane
+  // 0's value is not used), no LDS round trip
+  const uint32_t k1lo = (uint32_t)k1, k1hi = (uint32_t)(k1 >> 32);
+  const uint32_t p1lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k1lo, 0x138, 0xF, 0xF, false);
+  const uint32_t p1hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k1hi, 0x138, 0xF, 0xF, false);
+  const uint64_t p1 = ((uint64_t)p1hi << 32) | p1lo;
+  const uint32_t p2 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)k2, 0x138, 0xF, 0xF, false);
   const uint64_t onmask = __ballot(on);
   const bool prev_on = lane > 0 && ((onmask >> (lane - 1)) & 1ull);
   const bool start = on && !(prev_on && p1 == k1 && p2 == k2);
@@ -260,6 +265,13 @@ __device__ __forceinline__ bool leaf_claim(uint32_t* cp, uint32_t& seen) {
   if (v == kEmptyChild) v = cas32(cp, kEmptyChild, kPendingChild);
   seen = v;
   return v == kEmptyChild;
+}
+
+// leaf_claim for a caller that has just loaded the entry and found it EMPTY:
+// straight to the CAS (no second load round trip)
+__device__ __forceinline__ bool leaf_claim_empty(uint32_t* cp, uint32_t& seen) {
+  seen = cas32(cp, kEmptyChild, kPendingChild);
+  return seen == kEmptyChild;
 }
 
 // Leaf indices for the lanes that claimed an entry (allocateLeafGrid and
@@ -493,7 +505,7 @@ __device__ __forceinline__ void leaves_step(const MapView& m, LeafWalk& lw, Root
     const bool missing = j < n && v[j] == kEmptyChild;
     const int lead = run_leader(missing, (uint64_t)cp[j], 0u);
     uint32_t seen;
-    won[j] = (missing && lead == (int)lane_id()) ? leaf_claim(cp[j], seen) : false;
+    won[j] = (missing && lead == (int)lane_id()) ? leaf_claim_empty(cp[j], seen) : false;
   }
 #pragma unroll
   for (int j = 0; j < kLeafRun; ++j) leaf_publish(m, won[j], cp[j], (uint32_t)slot, lx[j], ly[j], lz[j]);
@@ -617,7 +629,7 @@ __device__ __forceinline__ uint32_t fresh_step(const MapView& m, RayWalk& w, Roo
   const int lead = run_leader(missing, (uint64_t)cp, 0u);
   bool won = false;
   uint32_t seen = entry;
-  if (missing && lead == (int)lane_id()) won = leaf_claim(cp, seen);
+  if (missing && lead == (int)lane_id()) won = leaf_claim_empty(cp, seen);
   const int32_t lmask = ~((1 << m.lb) - 1);
   const uint32_t idx = leaf_publish(m, won, cp, (uint32_t)slot, w.cx & lmask, w.cy & lmask, w.cz & lmask);
   if (won) seen = idx;
