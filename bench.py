@@ -309,7 +309,7 @@ def run_lookups(ctx, bx, scenes, reps=20):
         g.set_values(dc, dv)
     ctx.profile(False)
     set_kern = {k: ctx.profile_read(k)[1] / max(ctx.profile_read(k)[0], 1)
-                for k in ("k_op_clear", "k_op_resolve", "k_op_apply")}
+                for k in ("k_op_clear", "k_op_begin", "k_op_resolve", "k_zero_new_leaves", "k_op_apply")}
     set_ms = sum(set_kern.values())
     dq = ctx.to_device(q)
     out_v = ctx.array(len(q), np.float32)
@@ -1044,7 +1044,7 @@ def main():
         out["lookup"]["insert_roofline"] = {"bound": "hbm", "achieved": ins_gbs, "peak": HBM_PEAK_GBS,
                                             "unit": "GB/s", "frac": ins_gbs / HBM_PEAK_GBS,
                                             "bytes_per_insert": INSERT_BYTES,
-                                            "kernels": "k_op_clear + k_op_resolve + k_op_apply"}
+                                            "kernels": "k_op_clear + k_op_begin + k_op_resolve + k_zero_new_leaves + k_op_apply"}
     if c1 is not None:
         out["box_scan"] = c1
         out["lidar_sequence"] = c3

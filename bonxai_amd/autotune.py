@@ -15,11 +15,15 @@ benchmark's workload):
   leaf_window         the same in leaves for the leaf pass
   fresh_pass          maps empty at a scan's start: one create-and-mark walk
                       (1) or the leaf and update walks (0)
-by coordinate descent from the defaults, and records the fastest setting per
+  fresh_blocks_per_cu workgroups per CU of the create-and-mark walk
+  fresh_window        its lockstep window, in cells
+by coordinate descent (the fresh-pass and shared keys on frames into
+emptied maps, the workload; the update- and leaf-pass keys on frames
+re-inserted into filled maps) from the defaults, and records the fastest setting per
 device (architecture name + CU count) in bonxai_amd/tuning.json.
 Context(tuned=True) applies the recorded setting of its device.
 
-Usage: python -m bonxai_amd.autotune [--maps 64] [--passes 2] [--state fresh|steady] [--out PATH]
+Usage: python -m bonxai_amd.autotune [--maps 64] [--passes 2] [--state both|fresh|steady] [--out PATH]
 """
 import argparse
 import json
@@ -41,7 +45,15 @@ CANDIDATES = {
     "ray_window": [4, 8, 16, 32, 64, 1 << 20],
     "leaf_window": [8, 16, 32, 64, 1 << 20],
     "fresh_pass": [1, 0],
+    "fresh_blocks_per_cu": [2, 4, 6, 8, 12, 16],
+    "fresh_window": [4, 8, 16, 32, 64, 1 << 20],
 }
+# keys searched on each kind of frame: frames into empty maps (the workload)
+# run the fresh pass; frames re-inserted into filled maps the leaf and
+# update passes.  Keys of both are searched on fresh frames.
+FRESH_KEYS = ["ray_block", "ray_chunk", "ep_table_factor", "root_slack", "fresh_pass",
+              "fresh_blocks_per_cu", "fresh_window"]
+STEADY_KEYS = ["ray_blocks_per_cu", "leaf_blocks_per_cu", "ray_window", "leaf_window"]
 
 
 def device_key(ctx):
@@ -103,7 +115,7 @@ def frame_ms(ctx, bx, descs, npoints, maps, fresh=True, reps=5, warmup=2):
     return total / reps
 
 
-def search(ctx, bx, scans, passes=2, fresh=True, log=print):
+def search(ctx, bx, scans, passes=2, fresh=True, log=print, keys=None, start=None):
     max_range = 30.0
     maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
     dev = [ctx.to_device(p) for p, _ in scans]
@@ -113,6 +125,8 @@ def search(ctx, bx, scans, passes=2, fresh=True, log=print):
     empty(ctx, bx, maps)
     bx.insert_batch(ctx, descs)
     best = {k: ctx.get_tuning(k) for k in CANDIDATES}
+    if start:
+        best.update(start)
     apply(ctx, best)
 
     def measure():
@@ -123,6 +137,8 @@ def search(ctx, bx, scans, passes=2, fresh=True, log=print):
     log(f"default {best} {best_ms:.3f} ms")
     for p in range(passes):
         for key, values in CANDIDATES.items():
+            if keys is not None and key not in keys:
+                continue
             for v in values:
                 if v == best[key]:
                     continue
@@ -146,8 +162,10 @@ def search(ctx, bx, scans, passes=2, fresh=True, log=print):
 def main(argv=None):
     ap = argparse.ArgumentParser(description="autotune the batched scan launch")
     ap.add_argument("--maps", type=int, default=64)
-    ap.add_argument("--state", choices=["fresh", "steady"], default="fresh",
-                    help="time frames into emptied maps (fresh, the workload) or re-inserted ones")
+    ap.add_argument("--state", choices=["both", "fresh", "steady"], default="both",
+                    help="time frames into emptied maps (fresh, the workload), re-inserted ones, "
+                         "or both: the fresh-pass keys on fresh frames, then the update-pass keys on "
+                         "re-inserted ones")
     ap.add_argument("--passes", type=int, default=2)
     ap.add_argument("--out", default=TUNING_FILE)
     args = ap.parse_args(argv)
@@ -157,8 +175,19 @@ def main(argv=None):
     scans = scenes.batch_scans(nmaps=args.maps, max_range=30.0)
     ctx = bx.Context(0)
     t0 = time.time()
-    best, best_ms, default_ms, table = search(ctx, bx, scans, args.passes, fresh=args.state == "fresh",
-                                              log=lambda *a: print(*a, flush=True))
+    log = lambda *a: print(*a, flush=True)  # noqa: E731
+    extra = {}
+    if args.state == "both":
+        best, best_ms, default_ms, table = search(ctx, bx, scans, args.passes, fresh=True, log=log,
+                                                  keys=FRESH_KEYS)
+        best_st, st_ms, st_default, table_st = search(ctx, bx, scans, args.passes, fresh=False, log=log,
+                                                      keys=STEADY_KEYS, start=best)
+        best = best_st
+        table = table + table_st
+        extra = {"steady_frame_ms": st_ms, "default_steady_frame_ms": st_default}
+    else:
+        best, best_ms, default_ms, table = search(ctx, bx, scans, args.passes, fresh=args.state == "fresh",
+                                                  log=log)
     rec = {}
     try:
         with open(args.out) as f:
@@ -167,9 +196,10 @@ def main(argv=None):
         pass
     rec[device_key(ctx)] = {
         "settings": best, "frame_ms": best_ms, "default_frame_ms": default_ms,
-        "workload": f"{len(scans)} LiDAR scans into {len(scans)} {'empty' if args.state == 'fresh' else 'filled'} "
+        "workload": f"{len(scans)} LiDAR scans into {len(scans)} "
+                    f"{ {'fresh': 'empty', 'steady': 'filled', 'both': 'empty (then filled)'}[args.state] } "
                     f"maps, 0.1 m, 30 m, {sum(len(p) for p, _ in scans)} points, captured graph",
-        "evaluations": len(table), "seconds": time.time() - t0,
+        "evaluations": len(table), "seconds": time.time() - t0, **extra,
     }
     with open(args.out, "w") as f:
         json.dump(rec, f, indent=1, sort_keys=True)

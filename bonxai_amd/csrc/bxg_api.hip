@@ -110,6 +110,8 @@ struct bxg_ctx {
   uint32_t ray_window = 8;    // lockstep windows of the update pass (cells) and leaf pass (leaves)
   uint32_t leaf_window = 32;
   uint32_t fresh_pass = 1;    // maps empty at a scan's start: one create-and-mark walk
+  uint32_t fresh_blocks_per_cu = 6;  // workgroups per CU of that walk
+  uint32_t fresh_window = 16;        // its lockstep window (cells)
   std::vector<Prof> prof_pending;
   std::vector<hipEvent_t> ev_free;
   std::vector<hipEvent_t> snap_pool;  // free snapshot events (SnapEvent)
@@ -730,10 +732,16 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
                           ? gr_full
                           : std::max<uint32_t>(1, std::min<uint32_t>(cus * c->small_ray_blocks_per_cu, most8));
   const uint32_t gl = std::max<uint32_t>(1, std::min<uint32_t>(cus * c->leaf_blocks_per_cu, most));
+  // the fresh pass: its own workgroups per CU for large calls
+  const uint32_t gf_full = std::max<uint32_t>(1, cus * c->fresh_blocks_per_cu);
+  const uint32_t gf = grid_points >= (uint64_t)gf_full * rb
+                          ? gf_full
+                          : std::max<uint32_t>(1, std::min<uint32_t>(cus * c->small_ray_blocks_per_cu, most8));
   sc.chunk_max = c->ray_chunk;
   sc.win_cells = (int32_t)c->ray_window;
   sc.win_leaves = (int32_t)c->leaf_window;
   sc.fresh_pass = c->fresh_pass;
+  sc.win_fresh = (int32_t)c->fresh_window;
   // new-leaf pass: a wavefront per new leaf, grid-stride over them
   const uint32_t gm = std::max<uint32_t>(1, std::min<uint32_t>(cus * 8, gp));
   BXG_TRY(launch(c, "k_scan_begin", k_scan_begin, 1, kBlock, hdr, desc, sc));
@@ -747,13 +755,13 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
     BXG_TRY(launch(c, "k_scan_extra_leaves", k_scan_extra_leaves, nscans, kBlock, hdr, desc));
   BXG_TRY(launch(c, "k_scan_endpoints", k_scan_endpoints, gw, kBlock, desc, sc));
   if (c->counting) {
-    BXG_TRY(launch(c, "k_scan_fresh_rays", k_scan_fresh_rays<true>, gr, rb, desc, sc));
+    BXG_TRY(launch(c, "k_scan_fresh_rays", k_scan_fresh_rays<true>, gf, rb, desc, sc));
     BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<true>, gr, rb, desc, sc));
     if (any_extra)
       BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<true>, nscans, kBlock, hdr, desc));
     BXG_TRY(launch(c, "k_scan_new_endpoints", k_scan_new_endpoints<true>, gw, kBlock, desc, sc));
   } else {
-    BXG_TRY(launch(c, "k_scan_fresh_rays", k_scan_fresh_rays<false>, gr, rb, desc, sc));
+    BXG_TRY(launch(c, "k_scan_fresh_rays", k_scan_fresh_rays<false>, gf, rb, desc, sc));
     BXG_TRY(launch(c, "k_scan_rays", k_scan_rays<false>, gr, rb, desc, sc));
     if (any_extra)
       BXG_TRY(launch(c, "k_scan_extra_rays", k_scan_extra_rays<false>, nscans, kBlock, hdr, desc));
@@ -1346,6 +1354,8 @@ const TuningKey kTuning[] = {
     {"ray_window", &bxg_ctx::ray_window, 1, 1 << 20, 1},
     {"leaf_window", &bxg_ctx::leaf_window, 1, 1 << 20, 1},
     {"fresh_pass", &bxg_ctx::fresh_pass, 0, 1, 1},
+    {"fresh_blocks_per_cu", &bxg_ctx::fresh_blocks_per_cu, 1, 256, 1},
+    {"fresh_window", &bxg_ctx::fresh_window, 1, 1 << 20, 1},
 };
 
 const TuningKey* tuning_key(const char* key) {

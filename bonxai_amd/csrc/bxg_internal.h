@@ -167,6 +167,7 @@ struct ScanScratch {
   int32_t win_cells;    // lockstep window of the update pass, in cells (0: the default)
   int32_t win_leaves;   // lockstep window of the leaf pass, in leaves (0: the default)
   uint32_t fresh_pass;  // 1: the rays of maps empty at the scan's start take the fresh pass
+  int32_t win_fresh;    // lockstep window of the fresh pass, in cells (0: the update pass's)
   uint32_t* winners;    // flattened point indices of per-voxel winners, bucketed
   uint32_t* wkey;       // per point: bucket of a winner, kNone otherwise
   uint32_t* hist;       // [kMaxBuckets] winners per bucket (zero between calls)

@@ -1100,8 +1100,9 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
   constexpr bool kCounted = kMode == 2 || kMode == 4;
   constexpr bool kFreshPass = kMode >= 3;
   constexpr int kPass = kMode == 0 ? kPassLeaves : (kFreshPass ? kPassFresh : kPassUpdate);
+  const int32_t wcells = sc.win_cells > 0 ? sc.win_cells : kSyncWindowCells;
   const int32_t win = kMode == 0 ? (sc.win_leaves > 0 ? sc.win_leaves : kSyncWindowLeaves)
-                                 : (sc.win_cells > 0 ? sc.win_cells : kSyncWindowCells);
+                                 : (kFreshPass && sc.win_fresh > 0 ? sc.win_fresh : wcells);
   const uint32_t nreg = nunits >= kRegions * 8 * cmax ? kRegions : 1u;
   // chunk size: kChunk units when there is work for every wavefront, down
   // to one unit a lane on small lists

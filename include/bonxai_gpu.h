@@ -142,7 +142,9 @@ int bxg_set_counting(bxg_ctx* ctx, int on);
  * may run ahead of the slowest ray of its wavefront; "fresh_pass" (default
  * 1): the rays of a map that is empty when its scan starts create their
  * leaves and mark their cells in one walk (0: the leaf pass and the update
- * pass, as for other maps). Every setting gives the same maps. */
+ * pass, as for other maps); "fresh_blocks_per_cu" (default 6) and
+ * "fresh_window" (default 16), that walk's workgroups per CU and lockstep
+ * window in cells. Every setting gives the same maps. */
 int bxg_ctx_set_tuning(bxg_ctx* ctx, const char* key, int64_t value);
 int bxg_ctx_get_tuning(bxg_ctx* ctx, const char* key, int64_t* value);
 

@@ -457,7 +457,8 @@ def test_batch_mixed_empty_and_filled_maps(ctx):
     {"ray_block": 256, "ray_chunk": 1024, "ray_blocks_per_cu": 1, "leaf_blocks_per_cu": 1,
      "small_ray_blocks_per_cu": 1, "ray_window": 1, "leaf_window": 1 << 20},
     {"fresh_pass": 0},
-    {"fresh_pass": 1, "ray_window": 1, "ray_block": 64},
+    {"fresh_pass": 1, "ray_window": 1, "ray_block": 64, "fresh_blocks_per_cu": 1, "fresh_window": 1},
+    {"fresh_blocks_per_cu": 16, "fresh_window": 1 << 20, "ray_chunk": 64},
 ])
 def test_tuning_settings_keep_results(setting):
     """Every launch setting the autotuner may pick (workgroup size, rays per
