@@ -699,7 +699,16 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
   // Every lane of a live wavefront stays to the end: the root, leaf and
   // endpoint claims below are wavefront collectives.
   const bool live = gi < total;
-  const uint32_t s = live ? find_scan(sd, hdr->nscans, gi) : 0u;
+  // the scan of the wavefront's first point, searched once with uniform
+  // (scalar) loads; lanes past a scan boundary step forward from it
+  const uint32_t nscans = hdr->nscans;
+  const uint64_t g0 = (uint64_t)blockIdx.x * blockDim.x +
+                      ((uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)) << 6);
+  uint32_t s = find_scan(sd, nscans, g0);
+  if (live)
+    while (s + 1 < nscans && sd[s + 1].pt_off <= gi) ++s;
+  else
+    s = 0u;
   const ScanDesc& d = sd[s];
   const MapView m = load_view(d.map);
   int4 rec = make_int4(0, 0, 0, (int32_t)(s << 16));

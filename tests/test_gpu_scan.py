@@ -449,6 +449,32 @@ def test_batch_mixed_empty_and_filled_maps(ctx):
             d.free()
 
 
+def test_batch_of_more_scans_than_bucket_tags(ctx):
+    """A wave of 70 scans (more than the 64 bucket tags, so the winner list is
+    not split by pipeline and every walk skips the other pipeline's rays),
+    empty and filled maps mixed: every map equals the oracle's."""
+    import bonxai_amd as bx
+    from bonxai_amd import scenes
+    ora = _oracle()
+    scans = scenes.batch_scans(nmaps=70, lo=1_000, hi=6_000, seed=23)
+    maps = [bx.ProbabilisticMap(ctx, 0.1) for _ in scans]
+    models = [ora.ProbMap(0.1) for _ in scans]
+    for i in range(0, len(scans), 3):  # every third map holds a scan already
+        p, o = scans[(i + 1) % len(scans)]
+        maps[i].insert_point_cloud(p, o, 30.0)
+        models[i].insert_point_cloud(p, o, 30.0)
+    dev = [ctx.to_device(p) for p, _ in scans]
+    bx.insert_batch(ctx, [(m, d, o, 30.0) for m, d, (_, o) in zip(maps, dev, scans)])
+    for i, (m, om, (p, o)) in enumerate(zip(maps, models, scans)):
+        assert m.sync() == 0
+        om.insert_point_cloud(p, o, 30.0)
+        assert_same_cells(_gpu_cells(m), om.cells(), f"map {i}")
+        om.close()
+        m.close()
+    for d in dev:
+        d.free()
+
+
 @pytest.mark.parametrize("setting", [
     {"ray_block": 64, "ray_chunk": 64, "ray_blocks_per_cu": 16, "leaf_blocks_per_cu": 4,
      "small_ray_blocks_per_cu": 8},
