@@ -617,11 +617,13 @@ __device__ __forceinline__ void fresh_entry(const MapView& m, const RayWalk& w, 
 // leaf's touched bytes.  A leaf another wavefront is still creating
 // (PENDING) is re-read and the step taken again on the lane's next turn; its
 // creator publishes it within its own step.  A full root directory or pool
-// (the call fails) stops the ray.  Returns the cells marked first (counted
+// (the call fails) stops the ray.  The run that holds the endpoint's cell
+// stores its leaf index in *ep_leaf.  Returns the cells marked first (counted
 // form, exact, by atomics).
 template <bool kCount>
 __device__ __forceinline__ uint32_t fresh_step(const MapView& m, RayWalk& w, RootCache& rc,
-                                               int32_t& slot, uint32_t& entry, bool& stopped) {
+                                               int32_t& slot, uint32_t& entry, bool& stopped,
+                                               uint32_t* ep_leaf) {
   uint32_t* cp = slot >= 0 ? (uint32_t*)&m.children[(uint64_t)slot * m.isz +
                                                     inner_index(w.cx, w.cy, w.cz, m.lb, m.ib)]
                            : nullptr;
@@ -663,6 +665,9 @@ __device__ __forceinline__ uint32_t fresh_step(const MapView& m, RayWalk& w, Roo
       }
     }
   }
+  // the run holding the endpoint's cell (mx) leaves its leaf for
+  // k_scan_new_endpoints, which then needs no lookup
+  if (k0 <= w.mx && w.mx < k0 + (int32_t)n) *ep_leaf = seen;
   if (!w.done()) fresh_entry(m, w, rc, slot, entry);
   return nupd;
 }
@@ -1014,13 +1019,14 @@ enum : int { kPassLeaves = 0, kPassUpdate = 1, kPassFresh = 2 };
 __device__ __forceinline__ bool start_ray(const ScanDesc* __restrict__ sd, const ScanScratch& sc,
                                           uint32_t unit, uint32_t lg, MapView& m, RayWalk& w,
                                           RootCache& rc, int32_t& cur, uint32_t& lbase,
-                                          int pass, bool& first) {
+                                          int pass, bool& first, uint32_t& win) {
   // the leaf and fresh passes walk the endpoint's cell too (they create
   // leaves); the update pass only the ray's cells
   const bool leaf_pass = pass != kPassUpdate;
   const uint32_t idx = unit >> lg, seg = unit & ((1u << lg) - 1u);
   first = seg == 0;
   const uint32_t gi = sc.winners[idx];
+  win = gi;
   const int4 r = sc.rec[gi];
   const ScanDesc& d = sd[(uint32_t)r.w >> 16];
   const bool no_rays = (d.flags & kScanNoRays) != 0;
@@ -1134,6 +1140,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
   uint32_t entry = kEmptyChild;
   bool stopped = false;
   bool ray_first = false;
+  uint32_t wgi = 0;  // the ray's winner (its point index)
   // counted form: per-lane totals, flushed when the lane's map changes
   MapCounters* cmap = nullptr;
   uint64_t c_upd = 0, c_rays = 0, c_vis = 0;
@@ -1171,7 +1178,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       const uint32_t take = want < avail ? want : avail;
       const uint32_t rank = (uint32_t)__popcll(need & lanes_below());
       if (!have && rank < take) {
-        have = start_ray(sd, sc, chunk_pos + rank + (w0 << lg), lg, m, w, rc, cur, lbase, kPass, first);
+        have = start_ray(sd, sc, chunk_pos + rank + (w0 << lg), lg, m, w, rc, cur, lbase, kPass, first, wgi);
         nupd = 0;
         stopped = false;
         // the unit holds cell 0 of a ray with cells (the fresh pass also
@@ -1222,7 +1229,7 @@ __device__ void walk_winners(const ScanDesc* __restrict__ sd, const ScanScratch&
       if (kMode == 0) {
         leaves_step(m, lw, rc);
       } else if (kFreshPass) {
-        nupd += fresh_step<kCounted>(m, w, rc, slot, entry, stopped);
+        nupd += fresh_step<kCounted>(m, w, rc, slot, entry, stopped, &sc.tpos[wgi]);
       } else {
         nupd += apply_step<kCounted>(m, w, rc, cur, lbase, next_leaf);
       }
@@ -1302,7 +1309,8 @@ __global__ void __launch_bounds__(256) k_scan_endpoints(const ScanDesc* __restri
       const uint32_t lbase = *gp(&m.ctr->leaf_base);
       const bool fresh = leaf != kNone && leaf >= lbase;
       // in a map empty at the scan's start the fresh pass, which runs after
-      // this kernel, creates the endpoint's leaf: k_scan_new_endpoints finds it
+      // this kernel, creates the endpoint's leaf and stores its index here
+      // (k_scan_new_endpoints looks up any it did not)
       sc.tpos[gi] = fresh ? leaf : (leaf == kNone && lbase == 0 && !failed ? kLeafLater : kNone);
       if (leaf != kNone && !fresh) {
         const uint32_t li = leaf_index(r.x, r.y, r.z, m.lb);
