@@ -1305,13 +1305,14 @@ __global__ void __launch_bounds__(256) k_scan_endpoints(const ScanDesc* __restri
       const ScanDesc& d = sd[(uint32_t)r.w >> 16];
       const MapView m = load_view(d.map);
       const bool failed = (map_status(m) & kFatal) != 0;
-      const uint32_t leaf = failed ? kNone : find_leaf(m, r.x, r.y, r.z);
       const uint32_t lbase = *gp(&m.ctr->leaf_base);
+      // in a map empty at the scan's start every endpoint leaf is new: the
+      // fresh pass, which runs after this kernel, creates it and stores its
+      // index here (k_scan_new_endpoints looks up any it did not), so there
+      // is nothing to look up now
+      const uint32_t leaf = (failed || lbase == 0) ? kNone : find_leaf(m, r.x, r.y, r.z);
       const bool fresh = leaf != kNone && leaf >= lbase;
-      // in a map empty at the scan's start the fresh pass, which runs after
-      // this kernel, creates the endpoint's leaf and stores its index here
-      // (k_scan_new_endpoints looks up any it did not)
-      sc.tpos[gi] = fresh ? leaf : (leaf == kNone && lbase == 0 && !failed ? kLeafLater : kNone);
+      sc.tpos[gi] = fresh ? leaf : (lbase == 0 && !failed ? kLeafLater : kNone);
       if (leaf != kNone && !fresh) {
         const uint32_t li = leaf_index(r.x, r.y, r.z, m.lb);
         uint32_t BXG_G* cell = (uint32_t BXG_G*)m.cells + (uint64_t)leaf * m.lsz + li;
