@@ -1515,8 +1515,9 @@ int bxg_map_clear(bxg_map* m, int option) {
   // the old leaves' bytes stay: a leaf is written whole when it is created)
   BXG_TRY(fill64(c, m->h.root_keys, m->root_cap, kEmptyKey));
   BXG_TRY(fill32(c, m->h.children, (uint64_t)m->root_cap * m->h.inner_size, kEmptyChild));
-  const uint32_t z[3] = {0, 0, 0};  // leaf_count, root_count, status
-  BXG_HIP(c, hipMemcpyAsync(m->ctr, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+  static const uint32_t z[2] = {0, 0};  // leaf_count, root_count; then status
+  BXG_HIP(c, hipMemcpyAsync(&m->ctr->leaf_count, z, 8, hipMemcpyHostToDevice, c->stream));
+  BXG_HIP(c, hipMemcpyAsync(&m->ctr->status, z, 4, hipMemcpyHostToDevice, c->stream));
   m->extra.clear();
   BXG_TRY(ctx_sync(c));
   m->known_leaves = m->known_roots = 0;

@@ -31,6 +31,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 namespace bxg {
@@ -54,22 +55,33 @@ enum : uint32_t {
   kMapProbabilistic = 1u,
 };
 
-struct alignas(64) MapCounters {
+// Three 128-byte L2 lines: the bump counters, which the creating kernels
+// update with atomics; the words every kernel reads while a call runs
+// (status, cur, leaf_base: written only between kernels, or by a rare error
+// flag); the statistics, which some kernels add to.  A load of a line that
+// other lanes keep updating with atomics is served slowly, so the words the
+// kernels read stay off the atomically updated lines.
+struct alignas(128) MapCounters {
   uint32_t leaf_count;  // bump allocator (may run past leaf_cap on failure)
   uint32_t root_count;
+  uint32_t pad0[30];
   uint32_t status;
   int32_t cur;  // ProbabilisticMap update_count, 1..3
+  // leaf_count when the running scan or bulk call started: leaves at or
+  // above it were created by that call, and it initialises them
+  uint32_t leaf_base;
+  uint32_t pad1;
   uint64_t scans_applied;
+  uint64_t pad2[13];
   uint64_t stat_endpoint_updates;
   uint64_t stat_rays;
   uint64_t stat_ray_updates;
   uint64_t stat_ray_visits;  // counted runs: cells the scan's rays crossed
-  // leaf_count when the running scan or bulk call started: leaves at or
-  // above it were created by that call, and it initialises them
-  uint32_t leaf_base;
-  uint32_t pad;
+  uint64_t pad3[12];
 };
-static_assert(sizeof(MapCounters) == 64, "MapCounters is one cache line");
+static_assert(sizeof(MapCounters) == 384, "MapCounters is three cache lines");
+static_assert(offsetof(MapCounters, status) == 128 && offsetof(MapCounters, stat_endpoint_updates) == 256,
+              "MapCounters lines");
 
 struct alignas(64) MapDev {
   double inv_res;
