@@ -112,6 +112,7 @@ struct bxg_ctx {
   uint32_t fresh_pass = 1;    // maps empty at a scan's start: one create-and-mark walk
   uint32_t fresh_blocks_per_cu = 6;  // workgroups per CU of that walk
   uint32_t fresh_window = 16;        // its lockstep window (cells)
+  uint32_t ray_order = 1;            // winners ordered by length class, then direction
   std::vector<Prof> prof_pending;
   std::vector<hipEvent_t> ev_free;
   std::vector<hipEvent_t> snap_pool;  // free snapshot events (SnapEvent)
@@ -663,6 +664,8 @@ int build_desc(bxg_ctx* c, const bxg_scan_desc& in, uint32_t tag, uint64_t off, 
   d.fz = origin[2];
   d.max_range = in.max_range;
   d.r2 = in.max_range * in.max_range;  // probabilistic_map.hpp:145
+  const double reach = in.max_range * m->h.inv_res;
+  d.reach = reach >= 1.0 ? (uint32_t)std::min(reach, 16777216.0) : 0u;
   int32_t o[3];
   BXG_TRY(origin_voxel(m, origin, o));
   d.ox = o[0];
@@ -742,6 +745,7 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
   sc.win_leaves = (int32_t)c->leaf_window;
   sc.fresh_pass = c->fresh_pass;
   sc.win_fresh = (int32_t)c->fresh_window;
+  sc.ray_order = c->ray_order;
   // new-leaf pass: a wavefront per new leaf, grid-stride over them
   const uint32_t gm = std::max<uint32_t>(1, std::min<uint32_t>(cus * 8, gp));
   BXG_TRY(launch(c, "k_scan_begin", k_scan_begin, 1, kBlock, hdr, desc, sc));
@@ -1356,6 +1360,7 @@ const TuningKey kTuning[] = {
     {"fresh_pass", &bxg_ctx::fresh_pass, 0, 1, 1},
     {"fresh_blocks_per_cu", &bxg_ctx::fresh_blocks_per_cu, 1, 256, 1},
     {"fresh_window", &bxg_ctx::fresh_window, 1, 1 << 20, 1},
+    {"ray_order", &bxg_ctx::ray_order, 0, 1, 1},
 };
 
 const TuningKey* tuning_key(const char* key) {

@@ -860,8 +860,9 @@ __device__ __forceinline__ uint32_t wave_add_by_key(uint32_t* table, uint32_t ke
   return pos;
 }
 
-// octahedral direction bin of (dx, dy, dz), 64 x 64 bins in Morton order
-__device__ __forceinline__ uint32_t dir_bin(int32_t dx, int32_t dy, int32_t dz) {
+// octahedral direction bin of (dx, dy, dz), 2^b x 2^b bins in Morton order
+// (b = 5 or 6)
+__device__ __forceinline__ uint32_t dir_bin(int32_t dx, int32_t dy, int32_t dz, uint32_t b) {
   const float ax = fabsf((float)dx), ay = fabsf((float)dy), az = fabsf((float)dz);
   const float s = ax + ay + az;
   if (s == 0.f) return 0;
@@ -872,12 +873,14 @@ __device__ __forceinline__ uint32_t dir_bin(int32_t dx, int32_t dy, int32_t dz) 
     u = uu;
     v = vv;
   }
-  uint32_t iu = (uint32_t)((u + 1.f) * 32.f), iv = (uint32_t)((v + 1.f) * 32.f);
-  iu = iu > 63u ? 63u : iu;
-  iv = iv > 63u ? 63u : iv;
+  const float half = (float)(1u << (b - 1u));
+  const uint32_t top = (1u << b) - 1u;
+  uint32_t iu = (uint32_t)((u + 1.f) * half), iv = (uint32_t)((v + 1.f) * half);
+  iu = iu > top ? top : iu;
+  iv = iv > top ? top : iv;
   uint32_t key = 0;
 #pragma unroll
-  for (int b = 0; b < 6; ++b) key |= (((iu >> b) & 1u) << (2 * b)) | (((iv >> b) & 1u) << (2 * b + 1));
+  for (int k = 0; k < 6; ++k) key |= (((iu >> k) & 1u) << (2 * k)) | (((iv >> k) & 1u) << (2 * k + 1));
   return key;
 }
 
@@ -907,7 +910,21 @@ __global__ void __launch_bounds__(256) k_scan_winners(const BatchHdr* __restrict
         tag = ((fresh >> s) & 1ull) ? split + (uint32_t)__popcll(fresh & below)
                                     : (uint32_t)__popcll(~fresh & below);
       }
-      key = (tag << kDirBits) | dir_bin(r.x - d.ox, r.y - d.oy, r.z - d.oz);
+      const int32_t dx = r.x - d.ox, dy = r.y - d.oy, dz = r.z - d.oz;
+      uint32_t low;
+      if (sc.ray_order) {
+        // a ray's length class (its cell count mx against the max range, in
+        // quarters) above its direction: the rays of one wavefront end at
+        // about one distance (a batch waits for its longest ray) and, ending
+        // together, cross the same leaves (short rays of one direction hit
+        // near objects side by side, long ones the far field)
+        const uint32_t mx = (uint32_t)max(abs(dx), max(abs(dy), abs(dz)));
+        const uint32_t lc = d.reach ? min(3u, (uint32_t)(((uint64_t)mx * 4u) / (d.reach + 1u))) : 0u;
+        low = (lc << (kDirBits - 2u)) | dir_bin(dx, dy, dz, (kDirBits - 2u) / 2u);
+      } else {
+        low = dir_bin(dx, dy, dz, kDirBits / 2u);
+      }
+      key = (tag << kDirBits) | low;
     }
     sc.wkey[gi] = win ? key : kNone;
   }
