@@ -139,10 +139,10 @@ struct alignas(16) BatchHdr {
 
 // Winners are bucketed by (scan index mod 64, direction of the ray) so that
 // the rays one wavefront walks together touch the same cache lines; with
-// ScanScratch::ray_order the direction bins are 32 x 32 under 4 classes of
-// ray length (quarters of the scan's max range), so that a wavefront's rays
-// also end at about one distance.
-constexpr uint32_t kDirBits = 12;  // 64 x 64 octahedral direction bins, or 4 x 32 x 32
+// ScanScratch::ray_order = k the direction bins are 2^(6-k) x 2^(6-k) under
+// 4^k classes of ray length (equal parts of the scan's max range), so that a
+// wavefront's rays also end at about one distance.
+constexpr uint32_t kDirBits = 12;  // 64 x 64 octahedral direction bins, or 4^k x 2^(6-k) x 2^(6-k)
 constexpr uint32_t kMaxTagBits = 6;
 constexpr uint32_t kMaxBuckets = 1u << (kDirBits + kMaxTagBits);
 constexpr uint32_t kRegions = 8;   // winner-list regions, one per XCD
@@ -185,7 +185,7 @@ struct ScanScratch {
   int32_t win_leaves;   // lockstep window of the leaf pass, in leaves (0: the default)
   uint32_t fresh_pass;  // 1: the rays of maps empty at the scan's start take the fresh pass
   int32_t win_fresh;    // lockstep window of the fresh pass, in cells (0: the update pass's)
-  uint32_t ray_order;   // winner buckets: 0 direction only, 1 length class, then direction
+  uint32_t ray_order;   // winner buckets: 4^ray_order length classes, then direction (0..3)
   uint32_t* winners;    // flattened point indices of per-voxel winners, bucketed
   uint32_t* wkey;       // per point: bucket of a winner, kNone otherwise
   uint32_t* hist;       // [kMaxBuckets] winners per bucket (zero between calls)

@@ -861,7 +861,7 @@ __device__ __forceinline__ uint32_t wave_add_by_key(uint32_t* table, uint32_t ke
 }
 
 // octahedral direction bin of (dx, dy, dz), 2^b x 2^b bins in Morton order
-// (b = 5 or 6)
+// (b = 3..6)
 __device__ __forceinline__ uint32_t dir_bin(int32_t dx, int32_t dy, int32_t dz, uint32_t b) {
   const float ax = fabsf((float)dx), ay = fabsf((float)dy), az = fabsf((float)dz);
   const float s = ax + ay + az;
@@ -911,20 +911,16 @@ __global__ void __launch_bounds__(256) k_scan_winners(const BatchHdr* __restrict
                                     : (uint32_t)__popcll(~fresh & below);
       }
       const int32_t dx = r.x - d.ox, dy = r.y - d.oy, dz = r.z - d.oz;
-      uint32_t low;
-      if (sc.ray_order) {
-        // a ray's length class (its cell count mx against the max range, in
-        // quarters) above its direction: the rays of one wavefront end at
-        // about one distance (a batch waits for its longest ray) and, ending
-        // together, cross the same leaves (short rays of one direction hit
-        // near objects side by side, long ones the far field)
-        const uint32_t mx = (uint32_t)max(abs(dx), max(abs(dy), abs(dz)));
-        const uint32_t lc = d.reach ? min(3u, (uint32_t)(((uint64_t)mx * 4u) / (d.reach + 1u))) : 0u;
-        low = (lc << (kDirBits - 2u)) | dir_bin(dx, dy, dz, (kDirBits - 2u) / 2u);
-      } else {
-        low = dir_bin(dx, dy, dz, kDirBits / 2u);
-      }
-      key = (tag << kDirBits) | low;
+      // a ray's length class (its cell count mx against the max range, in
+      // 2^cb equal parts) above its direction: the rays of one wavefront end
+      // at about one distance (a batch waits for its longest ray) and, ending
+      // together, cross the same leaves (short rays of one direction hit near
+      // objects side by side, long ones the far field)
+      const uint32_t cb = 2u * sc.ray_order;
+      const uint32_t mx = (uint32_t)max(abs(dx), max(abs(dy), abs(dz)));
+      const uint32_t lc =
+          d.reach ? min((1u << cb) - 1u, (uint32_t)(((uint64_t)mx << cb) / (d.reach + 1u))) : 0u;
+      key = (tag << kDirBits) | (lc << (kDirBits - cb)) | dir_bin(dx, dy, dz, (kDirBits - cb) / 2u);
     }
     sc.wkey[gi] = win ? key : kNone;
   }

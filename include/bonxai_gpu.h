@@ -144,7 +144,9 @@ int bxg_set_counting(bxg_ctx* ctx, int on);
  * leaves and mark their cells in one walk (0: the leaf pass and the update
  * pass, as for other maps); "fresh_blocks_per_cu" (default 6) and
  * "fresh_window" (default 16), that walk's workgroups per CU and lockstep
- * window in cells. Every setting gives the same maps. */
+ * window in cells; "ray_order" (0..3, default 2): the rays of a call are
+ * walked in order of 4^ray_order classes of length (equal parts of the max
+ * range), then of direction. Every setting gives the same maps. */
 int bxg_ctx_set_tuning(bxg_ctx* ctx, const char* key, int64_t value);
 int bxg_ctx_get_tuning(bxg_ctx* ctx, const char* key, int64_t* value);
 

@@ -482,14 +482,16 @@ def test_batch_of_more_scans_than_bucket_tags(ctx):
      "leaf_window": 1},
     {"ray_block": 256, "ray_chunk": 1024, "ray_blocks_per_cu": 1, "leaf_blocks_per_cu": 1,
      "small_ray_blocks_per_cu": 1, "ray_window": 1, "leaf_window": 1 << 20},
-    {"fresh_pass": 0},
-    {"fresh_pass": 1, "ray_window": 1, "ray_block": 64, "fresh_blocks_per_cu": 1, "fresh_window": 1},
-    {"fresh_blocks_per_cu": 16, "fresh_window": 1 << 20, "ray_chunk": 64},
+    {"fresh_pass": 0, "ray_order": 0},
+    {"fresh_pass": 1, "ray_window": 1, "ray_block": 64, "fresh_blocks_per_cu": 1, "fresh_window": 1,
+     "ray_order": 2},
+    {"fresh_blocks_per_cu": 16, "fresh_window": 1 << 20, "ray_chunk": 64, "ray_order": 3},
 ])
 def test_tuning_settings_keep_results(setting):
     """Every launch setting the autotuner may pick (workgroup size, rays per
     lane, workgroups per CU, table load factors, lockstep windows, the fresh
-    pass on or off for maps empty at a scan's start) gives the same maps, in
+    pass on or off for maps empty at a scan's start, the order of the rays)
+    gives the same maps, in
     the batched call and in a captured graph."""
     import bonxai_amd as bx
     from bonxai_amd import scenes
