@@ -17,8 +17,8 @@ benchmark's workload):
                       (1) or the leaf and update walks (0)
   fresh_blocks_per_cu workgroups per CU of the create-and-mark walk
   fresh_window        its lockstep window, in cells
-  ray_order           winner order: 4^ray_order classes of ray length, then
-                      direction (2^(6-ray_order) bins a side)
+  ray_order           winner order: direction only (0), or classes of ray
+                      length and direction (1..5)
 by coordinate descent (the fresh-pass and shared keys on frames into
 emptied maps, the workload; the update- and leaf-pass keys on frames
 re-inserted into filled maps) from the defaults, and records the fastest setting per
@@ -49,7 +49,7 @@ CANDIDATES = {
     "fresh_pass": [1, 0],
     "fresh_blocks_per_cu": [2, 4, 6, 8, 12, 16],
     "fresh_window": [4, 8, 16, 32, 64, 1 << 20],
-    "ray_order": [2, 1, 3, 0],
+    "ray_order": [3, 4, 2, 5, 1, 0],
 }
 # keys searched on each kind of frame: frames into empty maps (the workload)
 # run the fresh pass; frames re-inserted into filled maps the leaf and

@@ -112,7 +112,7 @@ struct bxg_ctx {
   uint32_t fresh_pass = 1;    // maps empty at a scan's start: one create-and-mark walk
   uint32_t fresh_blocks_per_cu = 6;  // workgroups per CU of that walk
   uint32_t fresh_window = 16;        // its lockstep window (cells)
-  uint32_t ray_order = 2;            // winners ordered by 4^ray_order length classes, then direction
+  uint32_t ray_order = 3;            // winner order: direction sector, length class, direction
   std::vector<Prof> prof_pending;
   std::vector<hipEvent_t> ev_free;
   std::vector<hipEvent_t> snap_pool;  // free snapshot events (SnapEvent)
@@ -1360,7 +1360,7 @@ const TuningKey kTuning[] = {
     {"fresh_pass", &bxg_ctx::fresh_pass, 0, 1, 1},
     {"fresh_blocks_per_cu", &bxg_ctx::fresh_blocks_per_cu, 1, 256, 1},
     {"fresh_window", &bxg_ctx::fresh_window, 1, 1 << 20, 1},
-    {"ray_order", &bxg_ctx::ray_order, 0, 3, 1},
+    {"ray_order", &bxg_ctx::ray_order, 0, 5, 1},
 };
 
 const TuningKey* tuning_key(const char* key) {

@@ -139,10 +139,11 @@ struct alignas(16) BatchHdr {
 
 // Winners are bucketed by (scan index mod 64, direction of the ray) so that
 // the rays one wavefront walks together touch the same cache lines; with
-// ScanScratch::ray_order = k the direction bins are 2^(6-k) x 2^(6-k) under
-// 4^k classes of ray length (equal parts of the scan's max range), so that a
-// wavefront's rays also end at about one distance.
-constexpr uint32_t kDirBits = 12;  // 64 x 64 octahedral direction bins, or 4^k x 2^(6-k) x 2^(6-k)
+// ScanScratch::ray_order the direction bins are coarser and classes of ray
+// length (equal parts of the scan's max range) sit between a direction
+// sector and the direction inside it, so that a wavefront's rays also end at
+// about one distance (k_scan_winners, ray_order).
+constexpr uint32_t kDirBits = 12;  // direction and length-class bits of a bucket
 constexpr uint32_t kMaxTagBits = 6;
 constexpr uint32_t kMaxBuckets = 1u << (kDirBits + kMaxTagBits);
 constexpr uint32_t kRegions = 8;   // winner-list regions, one per XCD
@@ -152,9 +153,11 @@ static_assert((1u << kDirBits) % kScanTile == 0 && kTiles <= 64, "bucket scan ti
 // scan counter words: [0] winner count, [1, 1 + 6 * kRegions) region cursors
 // and exhausted flags of the three walks (leaf pass, update pass, fresh
 // pass), [kFreshWords, + 4) the fresh-pass split (below), [kTileWords, +
-// kTiles) bucket-tile sums
+// kTiles) bucket-tile sums, [kClipWords, + 64) per scan (index mod 64) the
+// points clipped at the max range among those of one wavefront in 16
 constexpr uint32_t kTileWords = 64;
-constexpr uint32_t kCounterWords = kTileWords + kTiles;
+constexpr uint32_t kClipWords = kTileWords + kTiles;
+constexpr uint32_t kCounterWords = kClipWords + 64;
 // With at most 2^kMaxTagBits scans (one bucket tag each), the scans whose map
 // is empty at the start take the highest tags, so their winners end the
 // list: [kFreshWords, +1] the 64-bit mask of those scans, [+2] the first of
@@ -185,7 +188,7 @@ struct ScanScratch {
   int32_t win_leaves;   // lockstep window of the leaf pass, in leaves (0: the default)
   uint32_t fresh_pass;  // 1: the rays of maps empty at the scan's start take the fresh pass
   int32_t win_fresh;    // lockstep window of the fresh pass, in cells (0: the update pass's)
-  uint32_t ray_order;   // winner buckets: 4^ray_order length classes, then direction (0..3)
+  uint32_t ray_order;   // winner order (0..5, k_scan_winners): 0 direction only
   uint32_t* winners;    // flattened point indices of per-voxel winners, bucketed
   uint32_t* wkey;       // per point: bucket of a winner, kNone otherwise
   uint32_t* hist;       // [kMaxBuckets] winners per bucket (zero between calls)

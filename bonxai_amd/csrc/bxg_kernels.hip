@@ -718,6 +718,7 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
   const MapView m = load_view(d.map);
   int4 rec = make_int4(0, 0, 0, (int32_t)(s << 16));
   bool valid = false;
+  bool clipped = false;
   int32_t ex = 0, ey = 0, ez = 0;
   uint32_t flags = kRecValid;
   if (live) {
@@ -756,6 +757,7 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
       if (d.flags & (kScanForceHit | kScanForceMiss)) {
         if (d.flags & kScanForceHit) flags |= kRecHit;
       } else if (sq >= d.r2) {
+        clipped = true;
         const double sr = __builtin_sqrt(sq);
         qx = d.fx + ((vx / sr) * d.max_range);
         qy = d.fy + ((vy / sr) * d.max_range);
@@ -840,6 +842,19 @@ __global__ void __launch_bounds__(256) k_scan_classify(const BatchHdr* __restric
   }
   if (valid) rec = make_int4(ex, ey, ez, (int32_t)(flags | (s << 16)));
   if (live) sc.rec[gi] = rec;
+  // points clipped at the max range, per scan, counted on one wavefront in
+  // 16 (one add per scan of the wavefront; the words share two cache lines,
+  // and an add from every wavefront measured 0.5 ms a 6.9 M-point frame):
+  // k_scan_winners orders the rays by length only in scans that mix such
+  // rays with returns
+  uint64_t pend = ((gi >> 6) & 15u) == 0 ? __ballot(clipped) : 0ull;
+  while (pend) {
+    const int leader = __ffsll((unsigned long long)pend) - 1;
+    const uint32_t ls = (uint32_t)__shfl((int)s, leader);
+    const uint64_t same = __ballot(clipped && s == ls);
+    if ((int)lane_id() == leader) add32(&sc.counters[kClipWords + (ls & 63u)], (uint32_t)__popcll(same));
+    pend &= ~same;
+  }
 }
 
 // Lanes of a wavefront that share a key form one group: one atomic add of
@@ -884,6 +899,23 @@ __device__ __forceinline__ uint32_t dir_bin(int32_t dx, int32_t dy, int32_t dz, 
   return key;
 }
 
+// Winner order of ScanScratch::ray_order: 2^b x 2^b direction bins, 2^cb
+// length classes, the top hb bits of the direction (Morton) above the class
+// (2b + cb = kDirBits)
+struct RayOrder {
+  uint32_t b, cb, hb;
+};
+__device__ __forceinline__ RayOrder ray_order(uint32_t k) {
+  switch (k) {
+    case 1: return {5u, 2u, 0u};
+    case 2: return {4u, 4u, 0u};
+    case 3: return {4u, 4u, 3u};
+    case 4: return {4u, 4u, 4u};
+    case 5: return {5u, 2u, 3u};
+    default: return {6u, 0u, 0u};
+  }
+}
+
 // Phase 2 (per point): the lowest-index point of each claimed voxel is its
 // winner (probabilistic_map.cpp:30-54 lets only the first point of a voxel
 // through the update_id test); winners are counted per bucket.
@@ -912,15 +944,26 @@ __global__ void __launch_bounds__(256) k_scan_winners(const BatchHdr* __restrict
       }
       const int32_t dx = r.x - d.ox, dy = r.y - d.oy, dz = r.z - d.oz;
       // a ray's length class (its cell count mx against the max range, in
-      // 2^cb equal parts) above its direction: the rays of one wavefront end
-      // at about one distance (a batch waits for its longest ray) and, ending
+      // 2^cb equal parts) between a coarse direction sector (hb bits) and
+      // its direction inside the sector: the rays of one wavefront end at
+      // about one distance (a batch waits for its longest ray) and, ending
       // together, cross the same leaves (short rays of one direction hit near
-      // objects side by side, long ones the far field)
-      const uint32_t cb = 2u * sc.ray_order;
+      // objects side by side, long ones the far field); the sectors keep each
+      // region of a one-scan list (one per XCD) on one part of the sphere
+      // with a share of every length
+      // (a scan with fewer than 1/8 of its points clipped at the max range
+      // (estimated from the wavefronts that counted them), such as a scan of
+      // a room, keeps the direction order: there neighbouring directions end
+      // at about one distance already, and the classes would only split its
+      // sweep of directions)
+      const bool mixed = (uint64_t)sc.counters[kClipWords + (s & 63u)] * 128u >= (uint64_t)d.n;
+      const RayOrder o = ray_order(mixed ? sc.ray_order : 0u);
+      const uint32_t db = 2u * o.b - o.hb;  // direction bits below the sector
       const uint32_t mx = (uint32_t)max(abs(dx), max(abs(dy), abs(dz)));
       const uint32_t lc =
-          d.reach ? min((1u << cb) - 1u, (uint32_t)(((uint64_t)mx << cb) / (d.reach + 1u))) : 0u;
-      key = (tag << kDirBits) | (lc << (kDirBits - cb)) | dir_bin(dx, dy, dz, (kDirBits - cb) / 2u);
+          d.reach ? min((1u << o.cb) - 1u, (uint32_t)(((uint64_t)mx << o.cb) / (d.reach + 1u))) : 0u;
+      const uint32_t dir = dir_bin(dx, dy, dz, o.b);
+      key = (tag << kDirBits) | ((dir >> db) << (o.cb + db)) | (lc << db) | (dir & ((1u << db) - 1u));
     }
     sc.wkey[gi] = win ? key : kNone;
   }
@@ -1596,6 +1639,7 @@ __global__ void k_scan_begin(const BatchHdr* __restrict__ hdr, const ScanDesc* _
     }
   }
   if ((nscans > 64u || nscans == 0u) && threadIdx.x == 0) sc.counters[kFreshWords + 2] = kNone;
+  if (threadIdx.x < 64u) sc.counters[kClipWords + threadIdx.x] = 0;
 }
 
 // Phase 5 (per scan): advance update_count 1 -> 2 -> 3 -> 1

@@ -144,9 +144,13 @@ int bxg_set_counting(bxg_ctx* ctx, int on);
  * leaves and mark their cells in one walk (0: the leaf pass and the update
  * pass, as for other maps); "fresh_blocks_per_cu" (default 6) and
  * "fresh_window" (default 16), that walk's workgroups per CU and lockstep
- * window in cells; "ray_order" (0..3, default 2): the rays of a call are
- * walked in order of 4^ray_order classes of length (equal parts of the max
- * range), then of direction. Every setting gives the same maps. */
+ * window in cells; "ray_order" (0..5, default 3): the order in which the rays
+ * of a scan are walked, by direction only (0) or by classes of length (equal
+ * parts of the max range) and direction: 1 and 2 are 4 classes over 32 x 32
+ * and 16 classes over 16 x 16 direction bins, 3 and 4 the latter under 8 and
+ * 16 direction sectors, 5 the former under 8 sectors; a scan with fewer than
+ * 1/8 of its points beyond the max range keeps the direction order. Every
+ * setting gives the same maps. */
 int bxg_ctx_set_tuning(bxg_ctx* ctx, const char* key, int64_t value);
 int bxg_ctx_get_tuning(bxg_ctx* ctx, const char* key, int64_t* value);
 

@@ -485,7 +485,8 @@ def test_batch_of_more_scans_than_bucket_tags(ctx):
     {"fresh_pass": 0, "ray_order": 0},
     {"fresh_pass": 1, "ray_window": 1, "ray_block": 64, "fresh_blocks_per_cu": 1, "fresh_window": 1,
      "ray_order": 2},
-    {"fresh_blocks_per_cu": 16, "fresh_window": 1 << 20, "ray_chunk": 64, "ray_order": 3},
+    {"fresh_blocks_per_cu": 16, "fresh_window": 1 << 20, "ray_chunk": 64, "ray_order": 5},
+    {"ray_order": 4, "fresh_pass": 0},
 ])
 def test_tuning_settings_keep_results(setting):
     """Every launch setting the autotuner may pick (workgroup size, rays per
