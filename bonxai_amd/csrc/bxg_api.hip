@@ -101,7 +101,7 @@ struct bxg_ctx {
   // pass of a call with fewer points than its tuned grid has lanes runs on
   // small_ray_blocks_per_cu workgroups per CU
   uint32_t ray_blocks_per_cu = 4;
-  uint32_t small_ray_blocks_per_cu = 4;
+  uint32_t small_ray_blocks_per_cu = 2;
   uint32_t leaf_blocks_per_cu = 16;
   uint32_t ray_block = 256;
   uint32_t ray_chunk = 256;
@@ -727,9 +727,9 @@ int enqueue_scan_kernels(bxg_ctx* c, const BatchHdr* hdr, const ScanDesc* desc, 
   const uint32_t most8 = (uint32_t)std::min<uint64_t>((grid_points * 8 + rb - 1) / rb, 1u << 30);
   // the tuned grid serves large calls (batched frames, lockstep batches); a
   // call with fewer points than that grid has lanes walks ray segments on a
-  // grid of small_ray_blocks_per_cu workgroups per CU (4 by default: 16
-  // queue on one cursor; short rays into a fresh map gain from 8, long rays
-  // into a grown map from 2)
+  // grid of small_ray_blocks_per_cu workgroups per CU (2 by default: more
+  // lanes cut the rays into more, shorter segments, and each segment starts
+  // with a seek and a root lookup of its own)
   const uint32_t gr_full = std::max<uint32_t>(1, cus * c->ray_blocks_per_cu);
   const uint32_t gr = grid_points >= (uint64_t)gr_full * rb
                           ? gr_full

@@ -135,8 +135,9 @@ int bxg_set_counting(bxg_ctx* ctx, int on);
 /* launch tuning of the context, set by the autotuner: "ray_blocks_per_cu"
  * and "leaf_blocks_per_cu" (1..256), the workgroups per CU of the
  * work-stealing update pass and leaf-allocation pass over the rays;
- * "small_ray_blocks_per_cu" (1..64, default 4), the update pass's workgroups
- * per CU for a call with fewer points than the tuned grid has lanes;
+ * "small_ray_blocks_per_cu" (1..64, default 2), the workgroups per CU of the
+ * update pass and the fresh pass of a call with fewer points than the tuned
+ * grid has lanes;
  * "ray_block", "ray_chunk", "ep_table_factor", "root_slack"; "ray_window"
  * (default 8) and "leaf_window" (default 32), how many cells / leaves a ray
  * may run ahead of the slowest ray of its wavefront; "fresh_pass" (default
