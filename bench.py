@@ -190,10 +190,7 @@ def run_batched(ctx, bx, scans, dev, steps, warmup, max_range, sets, steady=True
         lat_fresh.append(ctx.elapsed_ms(2, 3))
     check_maps(sets[0], "latency frames")
     # kernel durations, eager launches with per-kernel events
-    names = ("k_scan_begin", "k_scan_classify", "k_scan_winners", "k_scan_bucket_reduce",
-             "k_scan_bucket_scan", "k_scan_scatter", "k_scan_ray_leaves", "k_scan_endpoints",
-             "k_scan_fresh_rays", "k_scan_rays", "k_scan_new_endpoints", "k_scan_new_leaves",
-             "k_scan_finalize")
+    names = SCAN_KERNELS
     nprof = 3
     ctx.profile_reset()
     for _ in range(nprof):
@@ -396,6 +393,12 @@ def run_box_scan(ctx, bx, scenes, reps=10):
             "cpu_ms": cpu_s * 1e3, "cpu_rays_per_s": len(pts) / cpu_s, "cpu_cores": 1}
 
 
+SCAN_KERNELS = ("k_scan_begin", "k_scan_classify", "k_scan_winners", "k_scan_bucket_reduce",
+                "k_scan_bucket_scan", "k_scan_scatter", "k_scan_ray_leaves", "k_scan_endpoints",
+                "k_scan_fresh_rays", "k_scan_rays", "k_scan_new_endpoints", "k_scan_new_leaves",
+                "k_scan_finalize")
+
+
 def run_sequence(ctx, bx, scenes, nscans=200, cpu_budget_s=3.0):
     """Config 3: 200 LiDAR scans (64 x 1024 rays, 0.1 m, 50 m) into one map in
     order, one insert each (device points, pools reserved from an untimed
@@ -423,6 +426,20 @@ def run_sequence(ctx, bx, scenes, nscans=200, cpu_budget_s=3.0):
         raise RuntimeError("sequence failed")
     leaves = m.stats()["leaf_count"]
     m.close()
+    # the same sequence again with per-kernel events (after the timed run):
+    # microseconds a scan of each kernel
+    m = bx.ProbabilisticMap(ctx, 0.1)
+    m.reserve(int(st["leaf_count"] * 1.25) + 8192, int(st["root_count"] * 2) + 1024)
+    ctx.sync()
+    ctx.profile_reset()
+    ctx.profile(True)
+    for d, (_, o) in zip(dev, scans):
+        m.insert_point_cloud(d, o, 50.0, sync=False)
+    if m.sync() != 0:
+        raise RuntimeError("profiled sequence failed")
+    ctx.profile(False)
+    kern_us = {k: ctx.profile_read(k)[1] * 1e3 / nscans for k in SCAN_KERNELS}
+    m.close()
     for d in dev:
         d.free()
     npts = sum(len(p) for p, _ in scans)
@@ -439,6 +456,7 @@ def run_sequence(ctx, bx, scenes, nscans=200, cpu_budget_s=3.0):
     om.close()
     return {"scans": nscans, "points": npts, "ms": total_ms, "ms_per_scan": total_ms / nscans,
             "rays_per_s": npts / (total_ms / 1e3), "leaves": leaves,
+            "kernels_us_per_scan": {k: v for k, v in kern_us.items() if v > 0},
             "cpu_rays_per_s": cpu_pts / cpu_s, "cpu_cores": 1,
             "cpu_sample": f"the first {cpu_scans} scans ({cpu_pts} points, {cpu_s:.1f} s)"}
 
@@ -860,7 +878,9 @@ def main():
         c1 = run_box_scan(ctx, bx, scenes)
         log(f"box scan: {c1['ms']:.3f} ms ({c1['rays_per_s'] / 1e6:.0f} M rays/s), oracle {c1['cpu_ms']:.0f} ms")
         c3 = run_sequence(ctx, bx, scenes)
-        log(f"lidar sequence: {c3['ms_per_scan']:.3f} ms/scan ({c3['rays_per_s'] / 1e6:.0f} M rays/s)")
+        ku = c3["kernels_us_per_scan"]
+        log(f"lidar sequence: {c3['ms_per_scan']:.3f} ms/scan ({c3['rays_per_s'] / 1e6:.0f} M rays/s); "
+            f"update pass {ku.get('k_scan_rays', 0):.0f} us, leaf pass {ku.get('k_scan_ray_leaves', 0):.0f} us a scan")
         try:
             c3["dropin"] = run_dropin(scenes)
         except (RuntimeError, OSError, ValueError) as e:  # reported, not fatal to the benchmark
