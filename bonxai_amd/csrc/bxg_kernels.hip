@@ -943,19 +943,19 @@ __global__ void __launch_bounds__(256) k_scan_winners(const BatchHdr* __restrict
                                     : (uint32_t)__popcll(~fresh & below);
       }
       const int32_t dx = r.x - d.ox, dy = r.y - d.oy, dz = r.z - d.oz;
-      // a ray's length class (its cell count mx against the max range, in
-      // 2^cb equal parts) between a coarse direction sector (hb bits) and
-      // its direction inside the sector: the rays of one wavefront end at
-      // about one distance (a batch waits for its longest ray) and, ending
-      // together, cross the same leaves (short rays of one direction hit near
-      // objects side by side, long ones the far field); the sectors keep each
-      // region of a one-scan list (one per XCD) on one part of the sphere
-      // with a share of every length
-      // (a scan with fewer than 1/8 of its points clipped at the max range
-      // (estimated from the wavefronts that counted them), such as a scan of
-      // a room, keeps the direction order: there neighbouring directions end
-      // at about one distance already, and the classes would only split its
-      // sweep of directions)
+      // The ray's length class (its cell count mx against the max range, in
+      // 2^cb equal parts) sits between a coarse direction sector (hb bits)
+      // and the direction inside the sector.  The rays of one wavefront then
+      // end at about one distance (a batch waits for its longest ray) and,
+      // ending together, cross the same leaves: short rays of one direction
+      // hit near objects side by side, long ones sweep the far field.  The
+      // sectors keep each region of a one-scan list (one per XCD) on one part
+      // of the sphere, with a share of every length.
+      // A scan with under 1/8 of its points clipped at the max range (as
+      // estimated by the wavefronts that counted them), a scan of a room for
+      // one, keeps the direction order: there neighbouring directions end at
+      // about one distance already, and the classes would only split its
+      // sweep of directions.
       const bool mixed = (uint64_t)sc.counters[kClipWords + (s & 63u)] * 128u >= (uint64_t)d.n;
       const RayOrder o = ray_order(mixed ? sc.ray_order : 0u);
       const uint32_t db = 2u * o.b - o.hb;  // direction bits below the sector
