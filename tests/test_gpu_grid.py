@@ -51,6 +51,44 @@ def test_duplicates_last_write_wins(ctx):
     og.close()
 
 
+def test_set_values_without_was_on(ctx):
+    """setValue without was_on (the call keeps only each cell's last input):
+    repeated coordinates keep their last value, over three calls into float,
+    int32 and binary grids of several shapes, and an out-of-range coordinate
+    fails the call and leaves the cells unchanged."""
+    import bonxai_amd as bx
+    import oracle
+    rng = np.random.default_rng(7)
+    coords = rng.integers(-60, 60, (60_000, 3)).astype(np.int32)
+    ints = rng.integers(-10**9, 10**9, 60_000).astype(np.int32)
+    for dtype, ib, lb in ((np.int32, 2, 3), (np.float32, 1, 2), (np.int32, 4, 4), (None, 2, 3)):
+        vals = None if dtype is None else (ints if dtype == np.int32 else (ints % 1000).astype(np.float32))
+        g = bx.VoxelGrid(ctx, 0.1, ib, lb, dtype=dtype)
+        og = oracle.Grid(0.1, ib, lb, dtype=dtype)
+        for k in range(3):
+            sl = slice(k * 20_000, (k + 1) * 20_000)
+            assert g.set_values(coords[sl], None if vals is None else vals[sl]) is None
+            og.set_values(coords[sl], None if vals is None else vals[sl])
+        what = f"dtype {dtype}, bits {ib}/{lb}"
+        if dtype is None:
+            assert np.array_equal(sorted_cells(g.export()[0])[0], og.export()[0]), what
+        else:
+            assert_same_cells(g.export(), og.export(), what)
+        assert g.active_cells_count() == og.active_cells_count(), what
+        g.close()
+        og.close()
+    g = bx.VoxelGrid(ctx, 0.1, dtype=np.int32)
+    g.set_values(coords[:1000], ints[:1000])
+    before = sorted_cells(*g.export())
+    bad = np.concatenate([coords[1000:2000], np.array([[1 << 30, 0, 0]], np.int32)])
+    with pytest.raises(bx.BonxaiError) as e:
+        g.set_values(bad, ints[:1001])
+    assert e.value.code == bx.ERR_COORD_RANGE
+    g.clear_error()
+    assert_same_cells(g.export(), before, "a failed call")
+    g.close()
+
+
 def test_cells_on_off(ctx):
     import bonxai_amd as bx
     import oracle
